@@ -1,0 +1,237 @@
+"""ctypes binding of the FP64 CPU restatement -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product package never does.  The restatement itself is
+oracle/wfbp_oracle.cpp (reference line map in oracle/wfbp_oracle.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+
+class OrScan(C.Structure):
+    _fields_ = [("n_channels", C.c_int32), ("n_rows", C.c_int32),
+                ("n_views", C.c_int32), ("views_per_rotation", C.c_int32),
+                ("channel_spacing", C.c_double), ("row_spacing", C.c_double),
+                ("dsd", C.c_double), ("r", C.c_double),
+                ("central_channel", C.c_double), ("central_row", C.c_double),
+                ("angle0", C.c_double), ("z0", C.c_double), ("feed", C.c_double)]
+
+
+class OrRecon(C.Structure):
+    _fields_ = [("start_pos", C.c_double), ("end_pos", C.c_double),
+                ("recon_fov", C.c_double), ("slice_thickness", C.c_double),
+                ("slice_pitch", C.c_double), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("x_origin", C.c_double), ("y_origin", C.c_double),
+                ("filter_kind", C.c_int32), ("filter_c", C.c_double),
+                ("filter_a", C.c_double)]
+
+
+class OrDerived(C.Structure):
+    _fields_ = [("nfft", C.c_int32), ("csw", C.c_double), ("zrot", C.c_double),
+                ("theta_cone", C.c_double), ("acq_fov", C.c_double), ("ds", C.c_double),
+                ("half_acq_fov_sq", C.c_double), ("z_rot_over_2pi", C.c_double),
+                ("tan_half_cone", C.c_double), ("pixel_scale", C.c_double),
+                ("two_pi_over_ppr", C.c_double), ("views_per_half_turn", C.c_int32),
+                ("n_half_turns", C.c_int32), ("n_slices_native", C.c_int32),
+                ("n_slices_requested", C.c_int32)]
+
+
+class OrEllipsoid(C.Structure):
+    _fields_ = [("c", C.c_double * 3), ("a", C.c_double * 3), ("rot", C.c_double * 9),
+                ("dmu", C.c_double)]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        dp, fp, i64p, i32p = P(C.c_double), P(C.c_float), P(C.c_int64), P(C.c_int32)
+        sig = {
+            "or_geometry": (C.c_int, [P(OrScan), P(OrRecon), P(OrDerived)]),
+            "or_slice_grid": (C.c_int, [C.c_double, C.c_double, C.c_double, dp, C.c_int]),
+            "or_filter_kernel": (None, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, dp]),
+            "or_rebin": (None, [P(OrScan), fp, dp, C.c_int, C.c_int]),
+            "or_filter_rows": (None, [dp, C.c_int64, C.c_int, dp, C.c_int, C.c_int]),
+            "or_backproject": (C.c_int, [P(OrScan), P(OrRecon), dp, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, dp, i64p]),
+            "or_backproject_f32": (C.c_int, [P(OrScan), P(OrRecon), fp, C.c_int, C.c_int,
+                                             C.c_int, C.c_int, fp, i64p]),
+            "or_thicken": (C.c_int, [P(OrScan), P(OrRecon), dp, dp]),
+            "or_bp_counts": (C.c_int, [P(OrScan), P(OrRecon), C.c_int, C.c_int, i32p, i64p]),
+            "or_bp_selection": (C.c_int, [P(OrScan), P(OrRecon), C.c_int, C.c_int, C.c_int, i32p]),
+            "or_project_phantom": (None, [P(OrScan), C.c_double, C.c_double, P(OrEllipsoid),
+                                          C.c_int, fp, C.c_int, C.c_int]),
+            "or_reconstruct_native": (C.c_int, [P(OrScan), P(OrRecon), fp, C.c_int, C.c_int,
+                                                dp, i64p]),
+            "or_view_range": (None, [P(OrScan), P(OrRecon), C.c_int, C.c_int, P(C.c_int),
+                                     P(C.c_int)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray, ctype):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def scan_struct(scan) -> OrScan:
+    return OrScan(scan.n_channels, scan.n_rows, scan.n_views, scan.views_per_rotation,
+                  scan.channel_spacing, scan.row_spacing, scan.dsd, scan.r,
+                  scan.central_channel, scan.central_row, scan.angle0, scan.z0, scan.feed)
+
+
+def recon_struct(rc) -> OrRecon:
+    return OrRecon(rc.start_pos, rc.end_pos, rc.recon_fov, rc.slice_thickness,
+                   rc.slice_pitch, rc.nx, rc.ny, rc.x_origin, rc.y_origin,
+                   rc.filter_kind, rc.filter_c, rc.filter_a)
+
+
+def geometry(scan, rc=None) -> OrDerived:
+    d = OrDerived()
+    s = scan_struct(scan)
+    r = recon_struct(rc) if rc is not None else None
+    lib().or_geometry(C.byref(s), C.byref(r) if r is not None else None, C.byref(d))
+    return d
+
+
+def slice_grid(start, end, step) -> np.ndarray:
+    n = lib().or_slice_grid(start, end, step, None, 0)
+    z = np.zeros(max(n, 1), np.float64)
+    lib().or_slice_grid(start, end, step, _ptr(z, C.c_double), n)
+    return z[:n]
+
+
+def filter_kernel(n, ds, kind=0, c=1.0, a=1.0) -> np.ndarray:
+    h = np.zeros(n, np.float64)
+    lib().or_filter_kernel(n, ds, kind, c, a, _ptr(h, C.c_double))
+    return h
+
+
+def rebin(scan, raw: np.ndarray, v0=0, v1=None) -> np.ndarray:
+    v1 = scan.n_views if v1 is None else v1
+    raw = np.ascontiguousarray(raw, np.float32)
+    out = np.zeros((v1 - v0, scan.n_rows, scan.n_channels), np.float64)
+    s = scan_struct(scan)
+    lib().or_rebin(C.byref(s), _ptr(raw, C.c_float), _ptr(out, C.c_double), v0, v1)
+    return out
+
+
+def filter_rows(rows: np.ndarray, h: np.ndarray, nfft: int, use_fft=True) -> np.ndarray:
+    out = np.ascontiguousarray(rows, np.float64).copy()
+    ncols = out.shape[-1]
+    lib().or_filter_rows(_ptr(out, C.c_double), out.size // ncols, ncols,
+                         _ptr(np.ascontiguousarray(h, np.float64), C.c_double), nfft,
+                         1 if use_fft else 0)
+    return out
+
+
+def backproject(scan, rc, filtered: np.ndarray, v0: int, s0: int, s1: int):
+    filtered = np.ascontiguousarray(filtered, np.float64)
+    v1 = v0 + filtered.shape[0]
+    vol = np.zeros((s1 - s0, rc.ny, rc.nx), np.float64)
+    n = C.c_int64(0)
+    s, r = scan_struct(scan), recon_struct(rc)
+    rc_ = lib().or_backproject(C.byref(s), C.byref(r), _ptr(filtered, C.c_double), v0, v1,
+                               s0, s1, _ptr(vol, C.c_double), C.byref(n))
+    if rc_ != 0:
+        raise RuntimeError(f"or_backproject failed ({rc_})")
+    return vol, n.value
+
+
+def backproject_f32(scan, rc, filtered: np.ndarray, v0: int, s0: int, s1: int):
+    filtered = np.ascontiguousarray(filtered, np.float32)
+    v1 = v0 + filtered.shape[0]
+    vol = np.zeros((s1 - s0, rc.ny, rc.nx), np.float32)
+    n = C.c_int64(0)
+    s, r = scan_struct(scan), recon_struct(rc)
+    rc_ = lib().or_backproject_f32(C.byref(s), C.byref(r), _ptr(filtered, C.c_float), v0,
+                                   v1, s0, s1, _ptr(vol, C.c_float), C.byref(n))
+    if rc_ != 0:
+        raise RuntimeError(f"or_backproject_f32 failed ({rc_})")
+    return vol, n.value
+
+
+def thicken(scan, rc, native: np.ndarray) -> np.ndarray:
+    d = geometry(scan, rc)
+    native = np.ascontiguousarray(native, np.float64)
+    out = np.zeros((d.n_slices_requested, rc.ny, rc.nx), np.float64)
+    s, r = scan_struct(scan), recon_struct(rc)
+    lib().or_thicken(C.byref(s), C.byref(r), _ptr(native, C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def bp_counts(scan, rc, s0, s1):
+    counts = np.zeros((s1 - s0, rc.ny, rc.nx), np.int32)
+    n = C.c_int64(0)
+    s, r = scan_struct(scan), recon_struct(rc)
+    lib().or_bp_counts(C.byref(s), C.byref(r), s0, s1, _ptr(counts, C.c_int32), C.byref(n))
+    return counts, n.value
+
+
+def bp_selection(scan, rc, ix, iy, iz) -> np.ndarray:
+    sel = np.zeros(scan.n_views, np.int32)
+    s, r = scan_struct(scan), recon_struct(rc)
+    lib().or_bp_selection(C.byref(s), C.byref(r), ix, iy, iz, _ptr(sel, C.c_int32))
+    return sel
+
+
+def ellipsoid_array(phantom):
+    arr = (OrEllipsoid * max(1, len(phantom.ellipsoids)))()
+    for i, e in enumerate(phantom.ellipsoids):
+        arr[i].c[:] = list(e["c"])
+        arr[i].a[:] = list(e["a"])
+        arr[i].rot[:] = list(np.asarray(e["rot"], np.float64).reshape(9))
+        arr[i].dmu = e["dmu"]
+    return arr
+
+
+def project_phantom(scan, phantom, v0=0, v1=None) -> np.ndarray:
+    v1 = scan.n_views if v1 is None else v1
+    out = np.zeros((v1 - v0, scan.n_rows, scan.n_channels), np.float32)
+    arr = ellipsoid_array(phantom)
+    s = scan_struct(scan)
+    lib().or_project_phantom(C.byref(s), phantom.cyl_radius, phantom.cyl_mu, arr,
+                             len(phantom.ellipsoids), _ptr(out, C.c_float), v0, v1)
+    return out
+
+
+def view_range(scan, rc, s0, s1):
+    a, b = C.c_int(0), C.c_int(0)
+    s, r = scan_struct(scan), recon_struct(rc)
+    lib().or_view_range(C.byref(s), C.byref(r), s0, s1, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def reconstruct_native(scan, rc, raw: np.ndarray, s0: int, s1: int):
+    raw = np.ascontiguousarray(raw, np.float32)
+    vol = np.zeros((s1 - s0, rc.ny, rc.nx), np.float64)
+    n = C.c_int64(0)
+    s, r = scan_struct(scan), recon_struct(rc)
+    rc_ = lib().or_reconstruct_native(C.byref(s), C.byref(r), _ptr(raw, C.c_float), s0, s1,
+                                      _ptr(vol, C.c_double), C.byref(n))
+    if rc_ != 0:
+        raise RuntimeError(f"or_reconstruct_native failed ({rc_})")
+    return vol, n.value

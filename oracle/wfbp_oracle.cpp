@@ -1,0 +1,514 @@
+// wfbp_oracle.cpp -- TEST INFRASTRUCTURE ONLY: FP64 CPU restatement of the
+// FreeCT fct_wfbp path.  See wfbp_oracle.h for the reference line map.
+//
+// Index selection (which view / row / channel a voxel uses) is taken from the
+// shared FP32 arithmetic in freect_amd/csrc/wfbp_math.h so that it is
+// bit-identical to the GPU path; every weight, coordinate and sum that is not
+// a selection is recomputed here in FP64 from the geometry.  Build with
+// -ffp-contract=off (oracle/Makefile).
+#include "wfbp_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "../freect_amd/csrc/wfbp_math.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct Geo {
+  int ncols, nrows, nviews, ppr, H;
+  double dgamma, ds, dtheta, dz, kappa, csw, zrot, tanc, acq, hr;
+  FctBpConsts g;
+};
+
+Geo make_geo(const or_scan* s) {
+  Geo o{};
+  o.ncols = s->n_channels;
+  o.nrows = s->n_rows;
+  o.nviews = s->n_views;
+  o.ppr = s->views_per_rotation;
+  o.H = o.ppr / 2;
+  o.dgamma = s->channel_spacing / s->dsd;
+  o.ds = s->channel_spacing * s->r / s->dsd;          // RM.cu:491
+  o.dtheta = 2.0 * kPi / (double)o.ppr;
+  o.dz = s->feed / (double)o.ppr;
+  o.kappa = s->feed / (2.0 * kPi);
+  o.csw = s->r * s->row_spacing / s->dsd;             // RM.cu:62
+  o.zrot = std::fabs(s->feed);                        // RM.cu:64 (signed feed kept in kappa)
+  o.hr = 0.5 * ((double)o.nrows - 1.0);
+  o.tanc = o.hr * o.csw / s->r;                       // tan(atan(.)) of RM.cu:66-67
+  o.acq = 2.0 * s->r * std::sin(0.5 * ((double)o.ncols - 1.0) * o.dgamma);  // RM.cu:69
+  o.g = fct_tables::bp_consts(o.ncols, o.nrows, s->channel_spacing,
+                              s->row_spacing, s->dsd, s->r, s->central_channel,
+                              s->central_row, s->feed);
+  return o;
+}
+
+// r(t) of ReconstructionMachine.cu:493-498 / ramp.m:12-15.
+double ramp_r(double t) {
+  if (t == 0.0) return 0.5;
+  return std::sin(t) / t + (std::cos(t) - 1.0) / (t * t);
+}
+
+// Stierstorfer weight (cuda_kernels.cuh:79-93), FP64.
+double w_of_q(double q) {
+  const double Q = 0.6;
+  q = std::fabs(q);
+  if (q < Q) return 1.0;
+  if (q < 1.0) {
+    const double c = std::cos(0.5 * kPi * (q - Q) / (1.0 - Q));
+    return c * c;
+  }
+  return 0.0;
+}
+
+float w_of_q_f32(float q) {
+  q = fabsf(q);
+  if (q < FCT_WQ) return 1.0f;
+  if (q < 1.0f) {
+    const float c = cosf((0.5f * 3.14159265f) * (q - FCT_WQ) / (1.0f - FCT_WQ));
+    return c * c;
+  }
+  return 0.0f;
+}
+
+void fft_inplace(std::complex<double>* a, int n, bool inverse) {
+  for (int i = 1, j = 0; i < n; ++i) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    const double ang = 2.0 * kPi / (double)len * (inverse ? 1.0 : -1.0);
+    for (int i = 0; i < n; i += len) {
+      for (int k = 0; k < len / 2; ++k) {
+        const std::complex<double> w(std::cos(ang * k), std::sin(ang * k));
+        const std::complex<double> u = a[i + k];
+        const std::complex<double> v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+    }
+  }
+  if (inverse)
+    for (int i = 0; i < n; ++i) a[i] /= (double)n;
+}
+
+struct SliceGrid {
+  std::vector<double> zn, zr;   // native and requested slice z (FP64)
+};
+
+SliceGrid make_slices(const Geo& G, const or_recon* r) {
+  SliceGrid sg;
+  const double pitch = r->slice_pitch > 0 ? r->slice_pitch : r->slice_thickness;  // parse_config.cpp:59-60
+  const int nn = fct_tables::slice_count(r->start_pos, r->end_pos, G.csw);
+  const int nr = fct_tables::slice_count(r->start_pos, r->end_pos, pitch);
+  for (int i = 0; i < nn; ++i)
+    sg.zn.push_back(fct_tables::slice_z(r->start_pos, r->end_pos, G.csw, i));
+  for (int i = 0; i < nr; ++i)
+    sg.zr.push_back(fct_tables::slice_z(r->start_pos, r->end_pos, pitch, i));
+  return sg;
+}
+
+// Selection + FP64 (or FP32) weighting for native slices [s0, s1).
+// T = double: weights from FP64 geometry.  T = float: the FP32 CPU baseline.
+template <typename T, typename P>
+int bp_core(const or_scan* s, const or_recon* r, const P* filtered, int v0,
+            int v1, int s0, int s1, T* vol, int64_t* n_sel, int32_t* counts) {
+  const Geo G = make_geo(s);
+  const SliceGrid sg = make_slices(G, r);
+  const int nn = (int)sg.zn.size();
+  if (s0 < 0 || s1 > nn || s0 >= s1) return -1;
+  if (G.ppr % 2 != 0) return -2;
+  const int ns = s1 - s0;
+  const int nx = r->nx, ny = r->ny;
+  const int H = G.H;
+  const int chunk = fct_tables::kSliceChunk;
+
+  // FP32 tables (shared recipes) and FP64 geometry.
+  std::vector<float> xs(nx), ys(ny), c32(H), s32(H);
+  std::vector<double> x64(nx), y64(ny), c64(H), s64(H);
+  for (int i = 0; i < nx; ++i) {
+    xs[i] = fct_tables::voxel_coord(i, nx, r->recon_fov, r->x_origin);
+    x64[i] = ((double)i - 0.5 * ((double)nx - 1.0)) * (r->recon_fov / nx) + r->x_origin;
+  }
+  for (int i = 0; i < ny; ++i) {
+    ys[i] = fct_tables::voxel_coord(i, ny, r->recon_fov, r->y_origin);
+    y64[i] = ((double)i - 0.5 * ((double)ny - 1.0)) * (r->recon_fov / ny) + r->y_origin;
+  }
+  for (int t = 0; t < H; ++t) {
+    fct_tables::view_cos_sin(s->angle0, G.dtheta, t, &c32[t], &s32[t]);
+    c64[t] = std::cos(s->angle0 + G.dtheta * t);
+    s64[t] = std::sin(s->angle0 + G.dtheta * t);
+  }
+  std::vector<float> zs32(ns);
+  for (int i = 0; i < ns; ++i) {
+    const int si = s0 + i;
+    const double zref = sg.zn[(si / chunk) * chunk];
+    zs32[i] = (float)(sg.zn[si] - zref);
+  }
+  const int c_lo = s0 / chunk, c_hi = (s1 - 1) / chunk;
+  std::vector<float> zt32((size_t)(c_hi - c_lo + 1) * G.nviews);
+  for (int c = c_lo; c <= c_hi; ++c) {
+    const double zref = sg.zn[c * chunk];
+    for (int v = 0; v < G.nviews; ++v)
+      zt32[(size_t)(c - c_lo) * G.nviews + v] =
+          fct_tables::view_z_rel(s->z0, G.dz, v, zref);
+  }
+  const FctBpConsts g = G.g;
+  const double scale = 2.0 * kPi / (double)G.ppr;   // recon_structs.h:81
+  const size_t view_stride = (size_t)G.nrows * G.ncols;
+  int64_t total = 0;
+
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : total)
+  for (int col = 0; col < nx * ny; ++col) {
+    const int ix = col % nx, iy = col / nx;
+    std::vector<T> acc(ns, (T)0), num(ns), den(ns);
+    std::vector<int32_t> cnt(ns, 0);
+    if (fct_in_fov(xs[ix], ys[iy], g)) {
+      for (int t = 0; t < H; ++t) {
+        std::fill(num.begin(), num.end(), (T)0);
+        std::fill(den.begin(), den.end(), (T)0);
+        for (int k = 0;; ++k) {
+          const int v = t + k * H;
+          if (v >= G.nviews) break;
+          if (v < v0 || v >= v1) continue;
+          const float sg32 = (k & 1) ? -1.0f : 1.0f;
+          const FctBpColumn cl =
+              fct_bp_column(xs[ix], ys[iy], sg32 * c32[t], sg32 * s32[t], g);
+          // FP64 geometry of the same ray family.
+          const double sgn = (k & 1) ? -1.0 : 1.0;
+          const double cc = sgn * c64[t], ss = sgn * s64[t];
+          const double p = x64[ix] * ss - y64[iy] * cc;
+          const double l = std::sqrt(s->r * s->r - p * p) - (x64[ix] * cc + y64[iy] * ss);
+          const double beta = std::asin(p / s->r);
+          const double zsrc = s->z0 + G.dz * (double)v - G.kappa * beta;
+          const double pidx64 = p / G.ds + s->central_channel;
+          const P* view = filtered ? filtered + (size_t)(v - v0) * view_stride : nullptr;
+          for (int i = 0; i < ns; ++i) {
+            const float zt = zt32[(size_t)((s0 + i) / chunk - c_lo) * G.nviews + v];
+            if (!fct_bp_in_window(zt, zs32[i], g)) continue;
+            const float q32 = fct_bp_q(zs32[i], cl.invL, fct_bp_qoffset(cl, zt));
+            const float qi32 = fct_bp_qidx(q32, g);
+            int qf, pf;
+            if (!fct_bp_select(q32, qi32, cl.pidx, g, &qf, &pf)) continue;
+            T wq, wp, W;
+            if (sizeof(T) == sizeof(double)) {
+              const double q64 = (sg.zn[s0 + i] - zsrc) / (l * G.tanc);
+              const double qi64 = q64 * G.hr + s->central_row;
+              wq = (T)(qi64 - qf);
+              wp = (T)(pidx64 - pf);
+              W = (T)w_of_q(q64);
+            } else {
+              wq = (T)(qi32 - (float)qf);
+              wp = (T)(cl.pidx - (float)pf);
+              W = (T)w_of_q_f32(q32);
+            }
+            T val = (T)0;
+            if (view) {
+              const P* r0 = view + (size_t)qf * G.ncols + pf;
+              const P* r1 = r0 + G.ncols;
+              val = ((T)1 - wq) * (((T)1 - wp) * (T)r0[0] + wp * (T)r0[1]) +
+                    wq * (((T)1 - wp) * (T)r1[0] + wp * (T)r1[1]);
+            }
+            num[i] += W * val;
+            den[i] += W;
+            cnt[i] += 1;
+          }
+        }
+        for (int i = 0; i < ns; ++i)
+          if (den[i] > (T)0) acc[i] += num[i] / den[i];   // den = 0: no contribution
+      }
+    }
+    for (int i = 0; i < ns; ++i) {
+      if (vol) vol[(size_t)i * nx * ny + col] = (T)(scale * (double)acc[i]);
+      if (counts) counts[(size_t)i * nx * ny + col] = cnt[i];
+      total += cnt[i];
+    }
+  }
+  if (n_sel) *n_sel = total;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int or_geometry(const or_scan* s, const or_recon* r, or_derived* d) {
+  if (!s || !d) return -1;
+  const Geo G = make_geo(s);
+  std::memset(d, 0, sizeof(*d));
+  int n = 1;
+  while (n < 2 * G.ncols) n <<= 1;                    // RM.cu:54
+  d->nfft = n;
+  d->csw = G.csw;
+  d->zrot = G.zrot;
+  d->theta_cone = 2.0 * std::atan(G.hr * G.csw / s->r);
+  d->acq_fov = G.acq;
+  d->ds = G.ds;
+  d->half_acq_fov_sq = 0.25 * G.acq * G.acq;
+  d->z_rot_over_2pi = G.kappa;
+  d->tan_half_cone = std::tan(0.5 * d->theta_cone);
+  d->pixel_scale = s->dsd / (s->r * s->channel_spacing);
+  d->two_pi_over_ppr = 2.0 * kPi / (double)G.ppr;
+  d->views_per_half_turn = G.H;
+  d->n_half_turns = G.H > 0 ? G.nviews / G.H : 0;
+  if (r) {
+    const SliceGrid sg = make_slices(G, r);
+    d->n_slices_native = (int)sg.zn.size();
+    d->n_slices_requested = (int)sg.zr.size();
+  }
+  return 0;
+}
+
+int or_slice_grid(double start, double end, double step, double* z, int cap) {
+  const int n = fct_tables::slice_count(start, end, step);
+  for (int i = 0; i < n && i < cap; ++i) z[i] = fct_tables::slice_z(start, end, step, i);
+  return n;
+}
+
+void or_filter_kernel(int n, double ds, int kind, double c, double a, double* h) {
+  const double pi = kPi;
+  for (int i = -n / 2; i < n / 2; ++i) {
+    double v;
+    if (kind == 1) {
+      // Shepp & Logan (1974) band-limited kernel, same ds scaling as the ramp.
+      v = 2.0 / (pi * pi * ds * (1.0 - 4.0 * (double)i * (double)i));
+    } else {
+      const double t = c * pi * (double)i;              // RM.cu:503-505
+      v = (c * c / (2.0 * ds)) * (a * ramp_r(t) + ((1.0 - a) / 2.0) * ramp_r(pi * c * i + pi) +
+                                  ((1.0 - a) / 2.0) * ramp_r(pi * c * i - pi));
+    }
+    h[i + n / 2] = v;
+  }
+}
+
+void or_rebin(const or_scan* s, const float* raw, double* out, int v0, int v1) {
+  const Geo G = make_geo(s);
+  std::vector<fct_tables::RebinEntry> tab(G.ncols);
+  for (int c = 0; c < G.ncols; ++c)
+    tab[c] = fct_tables::rebin_entry(c, s->central_channel, G.dgamma, G.dtheta, G.ncols);
+  const size_t vs = (size_t)G.nrows * G.ncols;
+#pragma omp parallel for schedule(static)
+  for (int j = v0; j < v1; ++j) {
+    for (int row = 0; row < G.nrows; ++row) {
+      double* o = out + (size_t)(j - v0) * vs + (size_t)row * G.ncols;
+      for (int c = 0; c < G.ncols; ++c) {
+        const fct_tables::RebinEntry& e = tab[c];
+        if (!e.valid) { o[c] = 0.0; continue; }          // K:32-34
+        const int va = std::min(std::max(j + e.aoff, 0), G.nviews - 1);   // texture clamp
+        const int vb = std::min(std::max(j + e.aoff + 1, 0), G.nviews - 1);
+        const float* ra = raw + (size_t)va * vs + (size_t)row * G.ncols;
+        const float* rb = raw + (size_t)vb * vs + (size_t)row * G.ncols;
+        o[c] = (1.0 - e.fb) * ((1.0 - e.fa) * ra[e.b0] + e.fa * rb[e.b0]) +
+               e.fb * ((1.0 - e.fa) * ra[e.b1] + e.fa * rb[e.b1]);
+      }
+    }
+  }
+}
+
+void or_filter_rows(double* rows, int64_t nlines, int ncols, const double* h,
+                    int nfft, int use_fft) {
+  const int half = nfft / 2;
+  if (!use_fft) {
+#pragma omp parallel
+    {
+      std::vector<double> x(ncols);
+#pragma omp for schedule(static)
+      for (int64_t ln = 0; ln < nlines; ++ln) {
+        double* y = rows + ln * ncols;
+        std::copy(y, y + ncols, x.begin());
+        for (int i = 0; i < ncols; ++i) {
+          double acc = 0.0;
+          for (int j = 0; j < ncols; ++j) acc += h[i - j + half] * x[j];
+          y[i] = acc;
+        }
+      }
+    }
+    return;
+  }
+  // FFT form of the reference's R2C * H / N -> C2R (RM.cu:263-278): the data
+  // occupy ncols <= nfft/2 samples, so the circular product is the linear sum.
+  std::vector<std::complex<double>> Hs(nfft);
+  for (int n = 0; n < nfft; ++n) {
+    const int lag = n < half ? n : n - nfft;            // fftshift, RM.cu:509-513
+    Hs[n] = h[lag + half];
+  }
+  fft_inplace(Hs.data(), nfft, false);
+#pragma omp parallel
+  {
+    std::vector<std::complex<double>> buf(nfft);
+#pragma omp for schedule(static)
+    for (int64_t pr = 0; pr < (nlines + 1) / 2; ++pr) {
+      double* a = rows + (2 * pr) * ncols;
+      double* b = (2 * pr + 1 < nlines) ? rows + (2 * pr + 1) * ncols : nullptr;
+      std::fill(buf.begin(), buf.end(), std::complex<double>(0, 0));
+      for (int i = 0; i < ncols; ++i) buf[i] = std::complex<double>(a[i], b ? b[i] : 0.0);
+      fft_inplace(buf.data(), nfft, false);
+      for (int n = 0; n < nfft; ++n) buf[n] *= Hs[n].real();
+      fft_inplace(buf.data(), nfft, true);
+      for (int i = 0; i < ncols; ++i) {
+        a[i] = buf[i].real();
+        if (b) b[i] = buf[i].imag();
+      }
+    }
+  }
+}
+
+int or_backproject(const or_scan* s, const or_recon* r, const double* filtered,
+                   int v0, int v1, int s0, int s1, double* vol, int64_t* n_selected) {
+  return bp_core<double, double>(s, r, filtered, v0, v1, s0, s1, vol, n_selected, nullptr);
+}
+
+int or_backproject_f32(const or_scan* s, const or_recon* r, const float* filtered,
+                       int v0, int v1, int s0, int s1, float* vol, int64_t* n_selected) {
+  return bp_core<float, float>(s, r, filtered, v0, v1, s0, s1, vol, n_selected, nullptr);
+}
+
+int or_bp_counts(const or_scan* s, const or_recon* r, int s0, int s1, int32_t* counts,
+                 int64_t* n_selected) {
+  // Selection only: no filtered data is read.
+  const Geo G = make_geo(s);
+  return bp_core<double, double>(s, r, nullptr, 0, G.nviews, s0, s1, nullptr,
+                                 n_selected, counts);
+}
+
+int or_thicken(const or_scan* s, const or_recon* r, const double* native, double* out) {
+  const Geo G = make_geo(s);
+  const SliceGrid sg = make_slices(G, r);
+  const size_t plane = (size_t)r->nx * r->ny;
+  const int nr = (int)sg.zr.size(), nn = (int)sg.zn.size();
+#pragma omp parallel for schedule(static)
+  for (int so = 0; so < nr; ++so) {
+    std::vector<double> w(nn);
+    double wsum = 0.0;
+    for (int i = 0; i < nn; ++i) {
+      w[i] = std::max(0.0, 1.0 - std::fabs(sg.zn[i] - sg.zr[so]) / r->slice_thickness);  // K:303
+      wsum += w[i];
+    }
+    for (size_t p = 0; p < plane; ++p) {
+      double acc = 0.0;
+      for (int i = 0; i < nn; ++i)
+        if (w[i] != 0.0) acc += w[i] * native[(size_t)i * plane + p];
+      out[(size_t)so * plane + p] = wsum > 0.0 ? acc / wsum : 0.0;
+    }
+  }
+  return 0;
+}
+
+int or_bp_selection(const or_scan* s, const or_recon* r, int ix, int iy, int is,
+                    int32_t* sel) {
+  const Geo G = make_geo(s);
+  const SliceGrid sg = make_slices(G, r);
+  const int chunk = fct_tables::kSliceChunk;
+  const float x = fct_tables::voxel_coord(ix, r->nx, r->recon_fov, r->x_origin);
+  const float y = fct_tables::voxel_coord(iy, r->ny, r->recon_fov, r->y_origin);
+  const double zref = sg.zn[(is / chunk) * chunk];
+  const float zs = (float)(sg.zn[is] - zref);
+  const FctBpConsts g = G.g;
+  for (int v = 0; v < G.nviews; ++v) {
+    sel[v] = -1;
+    if (!fct_in_fov(x, y, g)) continue;
+    const int t = v % G.H, k = v / G.H;
+    float c, sn;
+    fct_tables::view_cos_sin(s->angle0, G.dtheta, t, &c, &sn);
+    const float sg32 = (k & 1) ? -1.0f : 1.0f;
+    const FctBpColumn cl = fct_bp_column(x, y, sg32 * c, sg32 * sn, g);
+    const float zt = fct_tables::view_z_rel(s->z0, G.dz, v, zref);
+    if (!fct_bp_in_window(zt, zs, g)) continue;
+    const float q = fct_bp_q(zs, cl.invL, fct_bp_qoffset(cl, zt));
+    int qf, pf;
+    if (fct_bp_select(q, fct_bp_qidx(q, g), cl.pidx, g, &qf, &pf)) sel[v] = qf * 65536 + pf;
+  }
+  return 0;
+}
+
+void or_project_phantom(const or_scan* s, double cyl_radius, double cyl_mu,
+                        const or_ellipsoid* e, int ne, float* out, int v0, int v1) {
+  const Geo G = make_geo(s);
+  const size_t vs = (size_t)G.nrows * G.ncols;
+#pragma omp parallel for schedule(static)
+  for (int v = v0; v < v1; ++v) {
+    const double a = s->angle0 + G.dtheta * (double)v;
+    const double sx = s->r * std::cos(a), sy = s->r * std::sin(a);
+    const double sz = s->z0 + G.dz * (double)v;
+    for (int row = 0; row < G.nrows; ++row) {
+      const double w = ((double)row - s->central_row) * s->row_spacing;
+      for (int c = 0; c < G.ncols; ++c) {
+        const double beta = ((double)c - s->central_channel) * G.dgamma;
+        // Ray from the source to its detector cell, unit direction u.
+        const double dx = -s->dsd * std::cos(a + beta);
+        const double dy = -s->dsd * std::sin(a + beta);
+        const double len = std::sqrt(s->dsd * s->dsd + w * w);
+        const double ux = dx / len, uy = dy / len, uz = w / len;
+        double acc = 0.0;
+        {  // infinite water cylinder along z
+          const double A = ux * ux + uy * uy;
+          const double B = 2.0 * (sx * ux + sy * uy);
+          const double C = sx * sx + sy * sy - cyl_radius * cyl_radius;
+          const double disc = B * B - 4.0 * A * C;
+          if (A > 0.0 && disc > 0.0) acc += cyl_mu * std::sqrt(disc) / A;
+        }
+        for (int i = 0; i < ne; ++i) {
+          const or_ellipsoid& E = e[i];
+          const double px = sx - E.c[0], py = sy - E.c[1], pz = sz - E.c[2];
+          // Body frame: R^T applied to the offset and the direction.
+          const double lx = (E.rot[0] * px + E.rot[3] * py + E.rot[6] * pz) / E.a[0];
+          const double ly = (E.rot[1] * px + E.rot[4] * py + E.rot[7] * pz) / E.a[1];
+          const double lz = (E.rot[2] * px + E.rot[5] * py + E.rot[8] * pz) / E.a[2];
+          const double mx = (E.rot[0] * ux + E.rot[3] * uy + E.rot[6] * uz) / E.a[0];
+          const double my = (E.rot[1] * ux + E.rot[4] * uy + E.rot[7] * uz) / E.a[1];
+          const double mz = (E.rot[2] * ux + E.rot[5] * uy + E.rot[8] * uz) / E.a[2];
+          const double A = mx * mx + my * my + mz * mz;
+          const double B = 2.0 * (lx * mx + ly * my + lz * mz);
+          const double C = lx * lx + ly * ly + lz * lz - 1.0;
+          const double disc = B * B - 4.0 * A * C;
+          if (disc > 0.0) acc += E.dmu * std::sqrt(disc) / A;
+        }
+        out[(size_t)(v - v0) * vs + (size_t)row * G.ncols + c] = (float)acc;
+      }
+    }
+  }
+}
+
+void or_view_range(const or_scan* s, const or_recon* r, int s0, int s1, int* v0, int* v1) {
+  const Geo G = make_geo(s);
+  const SliceGrid sg = make_slices(G, r);
+  double zmin = sg.zn[s0], zmax = sg.zn[s0];
+  for (int i = s0; i < s1; ++i) {
+    zmin = std::min(zmin, sg.zn[i]);
+    zmax = std::max(zmax, sg.zn[i]);
+  }
+  const double lo = zmin - G.zrot, hi = zmax + G.zrot;
+  double a, b;
+  if (G.dz > 0) { a = (lo - s->z0) / G.dz; b = (hi - s->z0) / G.dz; }
+  else if (G.dz < 0) { a = (hi - s->z0) / G.dz; b = (lo - s->z0) / G.dz; }
+  else { a = 0; b = G.nviews; }
+  int lo_v = (int)std::floor(a) - 2, hi_v = (int)std::ceil(b) + 3;
+  *v0 = std::max(0, lo_v);
+  *v1 = std::min(G.nviews, hi_v);
+}
+
+int or_reconstruct_native(const or_scan* s, const or_recon* r, const float* raw,
+                          int s0, int s1, double* vol, int64_t* n_selected) {
+  const Geo G = make_geo(s);
+  int v0, v1;
+  or_view_range(s, r, s0, s1, &v0, &v1);
+  const size_t vs = (size_t)G.nrows * G.ncols;
+  std::vector<double> par((size_t)(v1 - v0) * vs);
+  or_rebin(s, raw, par.data(), v0, v1);
+  or_derived d;
+  or_geometry(s, r, &d);
+  std::vector<double> h(d.nfft);
+  or_filter_kernel(d.nfft, G.ds, r->filter_kind, r->filter_c, r->filter_a, h.data());
+  or_filter_rows(par.data(), (int64_t)(v1 - v0) * G.nrows, G.ncols, h.data(), d.nfft, 1);
+  return or_backproject(s, r, par.data(), v0, v1, s0, s1, vol, n_selected);
+}
+
+}  // extern "C"

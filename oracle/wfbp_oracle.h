@@ -1,0 +1,101 @@
+// wfbp_oracle.h -- TEST INFRASTRUCTURE ONLY.
+//
+// FP64 CPU restatement of FreeCT's WFBP path (fct_wfbp), used by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
+// The product (freect_amd/) never links, loads or calls anything here.
+//
+// What follows which reference lines (paths relative to the FreeCT tree):
+//   or_geometry        ReconstructionMachine.cu:44-74, recon_structs.h:69-83
+//   or_slice_grid      ReconstructionMachine.cu:107-137
+//   or_filter_kernel   ReconstructionMachine.cu:478-513, resources/ramp.m:1-15
+//   or_rebin           cuda_kernels.cuh:11-39
+//   or_filter_rows     cuda_kernels.cuh:41-77 + ReconstructionMachine.cu:215-290
+//                      (FFT product == linear convolution, see below)
+//   or_backproject     cuda_kernels.cuh:79-93, 95-196, 198-287
+//   or_thicken         cuda_kernels.cuh:289-320
+//   or_project_phantom new (synthetic scan generator; analytic line integrals)
+//
+// Pins (tests/test_oracle_*.py): the nine reference filter vectors
+// (resources/filters/*.txt), Ram-Lak / Shepp-Logan closed forms, closed-form
+// geometry, W(q) special values, direct-vs-FFT convolution, rebinning of a
+// centred water cylinder against 2*mu*sqrt(a^2-t^2), FBP of a uniform cylinder
+// against mu, triangular thickening weights, analytic chord lengths.
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t n_channels, n_rows, n_views, views_per_rotation;
+  double channel_spacing;   // mm at the detector (arc length)
+  double row_spacing;       // mm at the detector
+  double dsd, r;            // source-detector, source-isocentre distance (mm)
+  double central_channel, central_row;
+  double angle0;            // tube angle of view 0 (rad)
+  double z0;                // source z of view 0 (mm)
+  double feed;              // table feed per rotation (mm, signed)
+} or_scan;
+
+typedef struct {
+  double start_pos, end_pos, recon_fov, slice_thickness, slice_pitch;
+  int32_t nx, ny;
+  double x_origin, y_origin;
+  int32_t filter_kind;      // 0: generalised-Hamming ramp (c, a); 1: Shepp-Logan
+  double filter_c, filter_a;
+} or_recon;
+
+typedef struct {
+  int32_t nfft;
+  double csw, zrot, theta_cone, acq_fov, ds;
+  double half_acq_fov_sq, z_rot_over_2pi, tan_half_cone, pixel_scale;
+  double two_pi_over_ppr;
+  int32_t views_per_half_turn, n_half_turns;
+  int32_t n_slices_native, n_slices_requested;
+} or_derived;
+
+typedef struct {
+  double c[3];       // centre (mm)
+  double a[3];       // semi-axes (mm)
+  double rot[9];     // row-major R; body axis i is column i of R
+  double dmu;        // additive attenuation (1/mm)
+} or_ellipsoid;
+
+int or_geometry(const or_scan* s, const or_recon* r, or_derived* d);
+int or_slice_grid(double start, double end, double step, double* z, int cap);
+void or_filter_kernel(int n, double ds, int kind, double c, double a, double* h);
+void or_rebin(const or_scan* s, const float* raw, double* out, int v0, int v1);
+void or_filter_rows(double* rows, int64_t nlines, int ncols, const double* h,
+                    int nfft, int use_fft);
+// filtered: parallel views [v0, v1) of [view][row][ch] FP64.  Computes native
+// slices [s0, s1) into vol (layout [s - s0][y][x]); counts selected samples.
+int or_backproject(const or_scan* s, const or_recon* r, const double* filtered,
+                   int v0, int v1, int s0, int s1, double* vol,
+                   int64_t* n_selected);
+// FP32-accumulation instantiation of the same loop (CPU baseline only).
+int or_backproject_f32(const or_scan* s, const or_recon* r,
+                       const float* filtered, int v0, int v1, int s0, int s1,
+                       float* vol, int64_t* n_selected);
+int or_thicken(const or_scan* s, const or_recon* r, const double* native,
+               double* out);
+// Selected-sample count per native voxel of slices [s0, s1) (layout like vol).
+int or_bp_counts(const or_scan* s, const or_recon* r, int s0, int s1,
+                 int32_t* counts, int64_t* n_selected);
+// One (voxel column, slice) of the selection rule, for bit-exact comparison:
+// writes, for every parallel view, qf*65536+pf or -1 when unused.
+int or_bp_selection(const or_scan* s, const or_recon* r, int ix, int iy,
+                    int is, int32_t* sel);
+void or_project_phantom(const or_scan* s, double cyl_radius, double cyl_mu,
+                        const or_ellipsoid* e, int ne, float* out, int v0,
+                        int v1);
+// Whole chain for native slices [s0, s1): raw fan data -> native volume.
+int or_reconstruct_native(const or_scan* s, const or_recon* r, const float* raw,
+                          int s0, int s1, double* vol, int64_t* n_selected);
+// View range [v0, v1) that native slices [s0, s1) can select.
+void or_view_range(const or_scan* s, const or_recon* r, int s0, int s1,
+                   int* v0, int* v1);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,295 @@
+"""Pins of the FP64 CPU restatement (oracle/) -- CPU only.
+
+Each test checks the restatement against something it does not itself
+compute: FreeCT's golden filter vectors, closed forms, invariants, or a brute
+force on tiny inputs.  Planted-mistake checks (a wrong parameter evaluated by
+the same code) show each pin is tight enough to catch a plausible error.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from freect_amd import scans as S
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+# FreeCT resources/filters/*.txt: (c, a) per file; ds from the sharp vector's
+# centre tap, h(0) = 1/(4 ds) for c = a = 1 (ReconstructionMachine.cu:493-506).
+FIXTURES = {
+    "672": {"sharp": (1, 1), "exp": (1, 1), "medium": (0.65, 0.5), "smooth": (0.4, 0.5)},
+    "736": {"sharp": (1, 1), "exp": (0.4, 0.5), "medium": (0.65, 0.5), "smooth": (0.4, 0.5)},
+    "920": {"sharp": (1, 1)},
+}
+
+
+def _fixture(nch, name):
+    return np.fromfile(os.path.join(GOLDEN, f"f_{nch}_{name}.txt"), "<f4")
+
+
+def _ulp_err(h32, f):
+    f = f.astype(np.float64)
+    big = np.abs(f) > 1e-6 * np.abs(f).max()
+    e = np.abs(h32.astype(np.float64) - f)
+    small = e[~big].max() if (~big).any() else 0.0
+    return (e[big] / np.spacing(np.abs(f[big]).astype(np.float32))).max(), small
+
+
+@pytest.mark.parametrize("nch,name", [(n, k) for n, d in FIXTURES.items() for k in d])
+def test_filter_matches_reference_vectors(oracle, nch, name):
+    sharp = _fixture(nch, "sharp")
+    n = len(sharp)
+    ds = 1.0 / (4.0 * float(sharp[n // 2]))
+    c, a = FIXTURES[nch][name]
+    f = _fixture(nch, name)
+    h = oracle.filter_kernel(n, ds, 0, c, a).astype(np.float32)
+    ulps, small = _ulp_err(h, f)
+    assert ulps <= 4, ulps
+    assert small < 1e-12
+
+
+def test_filter_planted_mistakes_fail(oracle):
+    sharp = _fixture("736", "sharp")
+    n = len(sharp)
+    ds = 1.0 / (4.0 * float(sharp[n // 2]))
+    f = _fixture("736", "medium")
+    for c, a in [(0.65, 0.54), (0.6, 0.5), (0.65, 1.0 - 0.5 + 0.01)]:
+        h = oracle.filter_kernel(n, ds, 0, c, a).astype(np.float32)
+        assert _ulp_err(h, f)[0] > 100
+
+
+def test_ramlak_closed_form(oracle):
+    n, ds = 1024, 0.7
+    h = oracle.filter_kernel(n, ds, 0, 1.0, 1.0)
+    k = np.arange(-n // 2, n // 2)
+    kk = np.where(k == 0, 1, k).astype(float)
+    ref = np.where(k == 0, 1.0 / (4 * ds), np.where(k % 2 != 0, -1.0 / (math.pi ** 2 * kk ** 2 * ds), 0.0))
+    assert np.allclose(h, ref, rtol=1e-12, atol=1e-14)
+    # Zero DC gain of the band-limited ramp: sum of taps -> 0 as n grows.
+    assert abs(h.sum()) < 2.0 / (math.pi ** 2 * ds * (n // 2))
+
+
+def test_shepp_logan_closed_form(oracle):
+    n, ds = 256, 0.5
+    h = oracle.filter_kernel(n, ds, 1)
+    k = np.arange(-n // 2, n // 2).astype(float)
+    assert np.allclose(h, 2.0 / (math.pi ** 2 * ds * (1 - 4 * k * k)), rtol=1e-13)
+    assert h[n // 2] > 0 and np.all(h[: n // 2] < 0)
+
+
+def test_hann_frequency_response(oracle):
+    # a = 0.5 (Hann window on the ramp, ramp.m:7-8): the DFT of the taps is
+    # |f| * (0.5 + 0.5 cos(pi f / f_c)) up to truncation of the spatial kernel.
+    n, ds = 4096, 1.0
+    h = oracle.filter_kernel(n, ds, 0, 1.0, 0.5)
+    Hf = np.real(np.fft.fft(np.fft.ifftshift(h)))
+    f = np.fft.fftfreq(n, d=ds)
+    ref = np.abs(f) * (0.5 + 0.5 * np.cos(2 * math.pi * f * ds))
+    band = np.abs(f) < 0.45 / ds
+    assert np.max(np.abs(Hf[band] - ref[band])) < 2e-3 * ref.max()
+
+
+def test_geometry_closed_forms(oracle):
+    w = S.full()
+    d = oracle.geometry(w.scan, w.recon)
+    sc = w.scan
+    assert d.nfft == 2048
+    assert abs(d.csw - 0.625) < 1e-12
+    assert abs(d.zrot - 39.36) < 1e-9
+    assert abs(d.acq_fov - 500.0) < 0.05  # dgamma 0.0010831 is quoted to 5 digits
+    assert abs(d.tan_half_cone - 31.5 * 0.625 / 541.0) < 1e-15
+    assert d.views_per_half_turn == 492 and d.n_half_turns == 20
+    assert abs(d.ds - 541.0 * 0.0010831) < 1e-12
+    assert d.n_slices_native == 320 and d.n_slices_requested == 320
+    assert abs(d.pixel_scale - 1.0 / d.ds) < 1e-12
+    ws = S.small()
+    ds_ = oracle.geometry(ws.scan, ws.recon)
+    assert ds_.nfft == 2048 and ds_.n_half_turns == 4
+    assert ds_.n_slices_native == 64 and ds_.n_slices_requested == 32
+    assert abs(ws.scan.feed - 10.0) < 1e-12
+
+
+def test_slice_grid(oracle):
+    z = oracle.slice_grid(-20.0, 20.0, 0.625)
+    assert len(z) == 64 and z[0] == -20.0 and abs(z[-1] - 19.375) < 1e-12
+    assert len(oracle.slice_grid(5.0, 5.0, 1.0)) == 1
+    zr = oracle.slice_grid(10.0, 0.0, 2.5)  # decreasing direction
+    assert np.allclose(zr, [10.0, 7.5, 5.0, 2.5])
+    assert len(oracle.slice_grid(0.0, 1.0, 0.3)) == 4  # 0, .3, .6, .9
+
+
+def test_w_special_values_through_backprojection(oracle):
+    # W(q) enters only via the weighted mean; a constant filtered sinogram
+    # must therefore backproject to scale * (number of covered theta~) exactly.
+    w = S.tiny(nx=16, n_slices=4)
+    sc, rc = w.scan, w.recon
+    d = oracle.geometry(sc, rc)
+    filt = np.ones((sc.n_views, sc.n_rows, sc.n_channels))
+    vol, n = oracle.backproject(sc, rc, filt, 0, 0, d.n_slices_native)
+    inside = vol != 0
+    assert n > 0
+    # Each covered theta~ contributes 2 pi / ppr * (sum W * 1 / sum W) = 2 pi / ppr.
+    ratio = vol[inside] / (2 * math.pi / sc.views_per_rotation)
+    assert np.allclose(ratio, np.round(ratio), atol=1e-9)
+    assert ratio.max() <= sc.views_per_rotation // 2
+
+
+def test_filter_fft_equals_direct(oracle):
+    rng = np.random.default_rng(3)
+    ncols, nfft = 150, 512
+    rows = rng.standard_normal((7, ncols))
+    h = oracle.filter_kernel(nfft, 0.8, 0, 0.65, 0.5)
+    a = oracle.filter_rows(rows, h, nfft, use_fft=False)
+    b = oracle.filter_rows(rows, h, nfft, use_fft=True)
+    assert np.max(np.abs(a - b)) < 1e-12 * np.abs(a).max()
+    # Direct form against numpy's linear convolution.
+    for i in range(rows.shape[0]):
+        full = np.convolve(rows[i], h)  # full[k] = sum_j rows[j] h[k - j]
+        ref = full[nfft // 2: nfft // 2 + ncols]
+        assert np.allclose(a[i], ref, rtol=1e-12, atol=1e-12)
+
+
+def test_rebin_centred_cylinder(oracle):
+    # A centred water cylinder gives view-independent fan data; rebinned
+    # parallel data must be 2 mu sqrt(a^2 - t^2) at t = (c - cc) ds.
+    w = S.tiny(n_channels=256, n_rows=4, ppr=256, rotations=1.0)
+    sc = w.scan.replace(feed=0.0, z0=0.0)
+    ph = S.Phantom(cyl_radius=120.0, ellipsoids=[])
+    raw = oracle.project_phantom(sc, ph)
+    par = oracle.rebin(sc, raw)
+    ds = sc.r * sc.dgamma
+    t = (np.arange(sc.n_channels) - sc.central_channel) * ds
+    # Rows see the cylinder along slanted rays: chord / cos(cone angle).
+    w_row = (np.arange(sc.n_rows) - sc.central_row) * sc.row_spacing
+    secant = np.sqrt(sc.dsd ** 2 + w_row ** 2) / sc.dsd
+    ref = 2 * S.MU_WATER * np.sqrt(np.maximum(120.0 ** 2 - t ** 2, 0.0))
+    ref = ref[None, :] * secant[:, None]
+    j = sc.n_views // 2
+    err = np.abs(par[j] - ref)
+    # Bilinear interpolation of a smooth profile: small, except at the edge kink.
+    interior = np.abs(t) < 115.0
+    assert err[:, interior].max() < 2e-3 * ref.max()
+    # Planted: a half-channel shift of the parallel grid is detected.
+    ref_shift = 2 * S.MU_WATER * np.sqrt(np.maximum(120.0 ** 2 - (t + 0.5 * ds) ** 2, 0.0))
+    assert np.abs(par[j] - ref_shift[None, :] * secant[:, None])[:, interior].max() > 4 * err[:, interior].max()
+
+
+def test_fbp_uniform_cylinder(oracle):
+    w = S.tiny(nx=48, n_slices=4, n_channels=192, n_rows=16, ppr=192, rotations=3)
+    sc, rc = w.scan, w.recon
+    ph = S.Phantom(cyl_radius=100.0, ellipsoids=[])
+    raw = oracle.project_phantom(sc, ph)
+    d = oracle.geometry(sc, rc)
+    vol, _ = oracle.reconstruct_native(sc, rc, raw, 0, d.n_slices_native)
+    x = (np.arange(rc.nx) - 0.5 * (rc.nx - 1)) * rc.recon_fov / rc.nx
+    X, Y = np.meshgrid(x, x)
+    inside = X ** 2 + Y ** 2 < 80 ** 2
+    outside = (X ** 2 + Y ** 2 > 120 ** 2) & (X ** 2 + Y ** 2 < 190 ** 2)
+    assert abs(vol[:, inside].mean() - S.MU_WATER) < 2e-3 * S.MU_WATER
+    assert np.abs(vol[:, outside]).max() < 0.05 * S.MU_WATER
+
+
+def test_thicken_triangular_weights(oracle):
+    w = S.tiny(nx=8, n_slices=8)
+    sc = w.scan
+    d0 = oracle.geometry(sc, w.recon)
+    rc = w.recon.replace(slice_thickness=2 * d0.csw, slice_pitch=2 * d0.csw)
+    d = oracle.geometry(sc, rc)
+    zn = oracle.slice_grid(rc.start_pos, rc.end_pos, d.csw)
+    zr = oracle.slice_grid(rc.start_pos, rc.end_pos, rc.slice_pitch)
+    native = np.stack([np.full((rc.ny, rc.nx), z) for z in zn])
+    out = oracle.thicken(sc, rc, native)
+    assert out.shape[0] == d.n_slices_requested == len(zr)
+    for s, z in enumerate(zr):
+        wts = np.maximum(0, 1 - np.abs(zn - z) / rc.slice_thickness)
+        assert np.allclose(out[s], (wts * zn).sum() / wts.sum(), rtol=1e-13)
+    # Grids here are aligned: weights (0.5, 1, 0.5) around each requested z.
+    assert np.allclose(out[1], zr[1], atol=1e-12)
+
+
+def test_phantom_chords_closed_form(oracle):
+    # A sphere of radius 30 at the isocentre, axial scan: the central ray's
+    # chord is 60 mm; a ray at parallel distance t has chord 2 sqrt(30^2 - t^2).
+    w = S.tiny(n_channels=64, n_rows=2, ppr=64, rotations=1.0)
+    sc = w.scan.replace(feed=0.0, z0=0.0, central_row=0.5)
+    e = {"c": (0.0, 0.0, 0.0), "a": (30.0, 30.0, 30.0), "rot": np.eye(3).reshape(9).tolist(),
+         "dmu": 1.0}
+    ph = S.Phantom(cyl_radius=1e-3, cyl_mu=0.0, ellipsoids=[e])
+    raw = oracle.project_phantom(sc, ph)
+    beta = (np.arange(sc.n_channels) - sc.central_channel) * sc.dgamma
+    t = sc.r * np.sin(beta)
+    w_row = (np.arange(2) - sc.central_row) * sc.row_spacing
+    # Slanted ray through the sphere centre plane: distance to centre shrinks by
+    # the in-plane/3D ratio; chord = 2 sqrt(R^2 - d^2) with d = t * cos(phi).
+    cosphi = sc.dsd / np.sqrt(sc.dsd ** 2 + w_row ** 2)
+    for r in range(2):
+        d = np.abs(t)
+        # The ray passes at height w*(R/Dsd) above the centre at closest approach.
+        h = w_row[r] * np.sqrt(sc.r ** 2 - t ** 2) / sc.dsd * 1.0
+        dist2 = (t ** 2) + (h * cosphi[r]) ** 2
+        ref = 2 * np.sqrt(np.maximum(900.0 - dist2, 0))
+        assert np.allclose(raw[0, r], ref, atol=2e-3)
+    # Rotated ellipsoid: the chord along a body axis equals twice that semi-axis.
+    R = S.euler_zyz(0.3, 1.1, -0.4)
+    e2 = {"c": (0, 0, 0), "a": (40.0, 20.0, 10.0), "rot": R.reshape(9).tolist(), "dmu": 1.0}
+    ph2 = S.Phantom(cyl_radius=1e-3, cyl_mu=0.0, ellipsoids=[e2])
+    # Mean line integral over views of a centred ellipsoid > 0 and bounded by 2*40.
+    raw2 = oracle.project_phantom(sc, ph2)
+    assert raw2.max() <= 80.0 + 1e-6 and raw2.max() > 20.0
+
+
+def test_phantom_mu_at_and_volume(oracle):
+    # Phantom.mu_at (used for error-vs-truth) agrees with the line integrals:
+    # integrating mu_at along one ray reproduces the projector's value.
+    w = S.tiny(n_channels=32, n_rows=2, ppr=32, rotations=1.0)
+    sc = w.scan
+    ph = S.make_phantom(5, -10, 10, n=4)
+    raw = oracle.project_phantom(sc, ph, 0, 1)
+    v, r, c = 0, 1, 13
+    a = sc.angle0
+    beta = (c - sc.central_channel) * sc.dgamma
+    src = np.array([sc.r * math.cos(a), sc.r * math.sin(a), sc.z0])
+    wv = (r - sc.central_row) * sc.row_spacing
+    dst = src + np.array([-sc.dsd * math.cos(a + beta), -sc.dsd * math.sin(a + beta), wv])
+    ts = (np.arange(200000) + 0.5) / 200000
+    pts = src[None, :] + ts[:, None] * (dst - src)[None, :]
+    integ = ph.mu_at(pts[:, 0], pts[:, 1], pts[:, 2]).mean() * np.linalg.norm(dst - src)
+    assert abs(integ - raw[0, r, c]) < 2e-3 * max(1.0, abs(raw[0, r, c]))
+
+
+def test_selection_is_fp32_rule(oracle):
+    # The oracle's per-view selection for one voxel agrees with an independent
+    # FP64 evaluation of the reference's rule except where the FP32 and FP64
+    # positions straddle a boundary (a handful of views at most).
+    w = S.tiny(nx=32, n_slices=8)
+    sc, rc = w.scan, w.recon
+    d = oracle.geometry(sc, rc)
+    zn = oracle.slice_grid(rc.start_pos, rc.end_pos, d.csw)
+    ix, iy, iz = 20, 9, 3
+    sel = oracle.bp_selection(sc, rc, ix, iy, iz)
+    x = (ix - 0.5 * (rc.nx - 1)) * rc.recon_fov / rc.nx
+    y = (iy - 0.5 * (rc.ny - 1)) * rc.recon_fov / rc.ny
+    H = sc.views_per_rotation // 2
+    mism = 0
+    used = 0
+    for v in range(sc.n_views):
+        t, k = v % H, v // H
+        th = sc.angle0 + 2 * math.pi * t / sc.views_per_rotation
+        c, s = math.cos(th) * (-1) ** k, math.sin(th) * (-1) ** k
+        p = x * s - y * c
+        l = math.sqrt(sc.r ** 2 - p ** 2) - (x * c + y * s)
+        zt = sc.z_of_view(v)
+        if abs(zt - zn[iz]) > abs(sc.feed):
+            ref = -1
+        else:
+            q = (zn[iz] - zt + sc.feed / (2 * math.pi) * math.asin(p / sc.r)) / (l * d.tan_half_cone)
+            qi = q * 0.5 * (sc.n_rows - 1) + sc.central_row
+            pi_ = p / d.ds + sc.central_channel
+            qf, pf = math.floor(qi), math.floor(pi_)
+            ok = abs(q) <= 1 and 0 <= qf <= sc.n_rows - 2 and 0 <= pf <= sc.n_channels - 2
+            ref = qf * 65536 + pf if ok else -1
+        used += ref >= 0
+        mism += ref != sel[v]
+    assert used > 20
+    assert mism <= 2
