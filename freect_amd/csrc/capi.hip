@@ -1,0 +1,626 @@
+// capi.hip -- host side of the C ABI: device context, plan setup (geometry,
+// FP64 tables, filter spectrum) and the stage entry points.
+//
+// Setup follows ConfigureCTGeometry / AllocateKeyArrays / generate_filter
+// (ReconstructionMachine.cu:44-139, 478-544) and GPUPrecompute
+// (recon_structs.h:69-83), with three changes of form: view angles, table
+// positions and slice positions are computed in FP64 and handed to the
+// kernels as FP32 values relative to a per-chunk FP64 reference; the filter
+// spectrum is computed in FP64 on the host; every failure is a status code.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "wfbp_internal.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+void set_err(fct_wfbp_gpu_ctx* ctx, const char* fmt, ...) {
+  if (!ctx) return;
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  ctx->last_error = buf;
+}
+
+#define HIP_TRY(ctx, expr)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      set_err(ctx, "%s failed: %s", #expr, hipGetErrorString(e_));                \
+      return e_ == hipErrorOutOfMemory ? FCT_ERR_OUT_OF_MEMORY : FCT_ERR_DEVICE;  \
+    }                                                                             \
+  } while (0)
+
+template <typename T>
+hipError_t upload(T** dst, const std::vector<T>& src, hipStream_t s) {
+  if (src.empty()) {
+    *dst = nullptr;
+    return hipSuccess;
+  }
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), src.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+void fft64(std::vector<std::complex<double>>& a) {
+  const int n = (int)a.size();
+  for (int i = 1, j = 0; i < n; ++i) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    for (int i = 0; i < n; i += len)
+      for (int k = 0; k < len / 2; ++k) {
+        const double ang = -2.0 * kPi * (double)k / (double)len;
+        const std::complex<double> w(std::cos(ang), std::sin(ang));
+        const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+  }
+}
+
+// Spatial filter taps h[i + n/2], i in [-n/2, n/2).  kind 0: the generalised
+// Hamming ramp of generate_filter (ReconstructionMachine.cu:493-506, ramp.m);
+// kind 1: the Shepp-Logan kernel 2 / (pi^2 ds (1 - 4 i^2)).
+std::vector<double> filter_taps(int n, double ds, int kind, double c, double a) {
+  auto r = [](double t) {
+    if (t == 0.0) return 0.5;
+    return std::sin(t) / t + (std::cos(t) - 1.0) / (t * t);
+  };
+  std::vector<double> h(n);
+  for (int i = -n / 2; i < n / 2; ++i) {
+    double v;
+    if (kind == 1) {
+      v = 2.0 / (kPi * kPi * ds * (1.0 - 4.0 * (double)i * (double)i));
+    } else {
+      v = (c * c / (2.0 * ds)) * (a * r(c * kPi * i) + ((1.0 - a) / 2.0) * r(kPi * c * i + kPi) +
+                                   ((1.0 - a) / 2.0) * r(kPi * c * i - kPi));
+    }
+    h[i + n / 2] = v;
+  }
+  return h;
+}
+
+int validate(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* s, const fct_recon_config* r,
+             const fct_exec_options* o) {
+  if (!s || !r) return set_err(ctx, "null geometry or config"), FCT_ERR_INVALID_ARGUMENT;
+  if (s->n_channels < 2 || s->n_rows < 2 || s->n_views < 1)
+    return set_err(ctx, "detector needs >= 2 channels, >= 2 rows and >= 1 view"),
+           FCT_ERR_INVALID_ARGUMENT;
+  if (s->n_channels > 2048)
+    return set_err(ctx, "n_channels %d exceeds 2048 (filter length limit)", s->n_channels),
+           FCT_ERR_LIMIT;
+  if (s->views_per_rotation < 2 || (s->views_per_rotation & 1))
+    return set_err(ctx, "views_per_rotation must be even and >= 2 (got %d)",
+                   s->views_per_rotation),
+           FCT_ERR_UNSUPPORTED;
+  if (!(s->dsd > s->r) || !(s->r > 0) || !(s->channel_spacing > 0) || !(s->row_spacing > 0))
+    return set_err(ctx, "invalid distances or spacings"), FCT_ERR_INVALID_ARGUMENT;
+  if (!(std::fabs(s->feed) > 0))
+    return set_err(ctx, "table feed must be nonzero (helical scans only)"), FCT_ERR_UNSUPPORTED;
+  if (r->nx < 1 || r->ny < 1 || r->nx > 8192 || r->ny > 8192)
+    return set_err(ctx, "nx, ny must be in [1, 8192]"), FCT_ERR_INVALID_ARGUMENT;
+  if (!(r->recon_fov > 0) || !(r->slice_thickness > 0))
+    return set_err(ctx, "recon_fov and slice_thickness must be > 0"), FCT_ERR_INVALID_ARGUMENT;
+  if (r->filter_kind < 0 || r->filter_kind > 1)
+    return set_err(ctx, "unknown filter_kind %d", r->filter_kind), FCT_ERR_INVALID_ARGUMENT;
+  if (o) {
+    if (o->precision < 0 || o->precision > 2)
+      return set_err(ctx, "unknown precision %d", o->precision), FCT_ERR_INVALID_ARGUMENT;
+    if (o->tile_z != 0 && o->tile_z != 8 && o->tile_z != 16 && o->tile_z != 32)
+      return set_err(ctx, "tile_z must be 0, 8, 16 or 32"), FCT_ERR_INVALID_ARGUMENT;
+  }
+  return FCT_OK;
+}
+
+void free_plan_buffers(fct_wfbp_gpu_plan* p) {
+  void* bufs[] = {p->d_xs,       p->d_ys,        p->d_cs,        p->d_zs,       p->d_zt,
+                  p->d_slab_vrange, p->d_rebin,  p->d_Hn,        p->d_tw,       p->d_thk_off,
+                  p->d_thk_idx,  p->d_thk_w,     p->d_filtered,  p->d_row_scale, p->d_native,
+                  p->d_raw_stage, p->d_vol_stage};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+}
+
+fct::BpArgs bp_args(fct_wfbp_gpu_plan* p) {
+  fct::BpArgs a{};
+  a.xs = p->d_xs;
+  a.ys = p->d_ys;
+  a.cs = p->d_cs;
+  a.zs = p->d_zs;
+  a.zt = p->d_zt;
+  a.slab_vrange = p->d_slab_vrange;
+  a.g = p->g;
+  a.scale = p->scale;
+  a.nx = p->recon.nx;
+  a.ny = p->recon.ny;
+  a.n_native = p->n_native;
+  a.nviews = p->nviews;
+  a.H = p->H;
+  a.cpitch = p->cpitch;
+  a.tiles_x = (p->recon.nx + 7) / 8;
+  a.tiles_y = (p->recon.ny + 7) / 8;
+  a.nslabs = p->nslabs;
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+FCT_API int fct_wfbp_gpu_create(int32_t device, fct_wfbp_gpu_ctx** out) {
+  if (!out) return FCT_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto* ctx = new (std::nothrow) fct_wfbp_gpu_ctx();
+  if (!ctx) return FCT_ERR_OUT_OF_MEMORY;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    delete ctx;
+    return FCT_ERR_DEVICE;
+  }
+  if (device < 0 || device >= n) {
+    delete ctx;
+    return FCT_ERR_INVALID_ARGUMENT;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return FCT_ERR_DEVICE;
+  }
+  ctx->device = device;
+  *out = ctx;
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_destroy(fct_wfbp_gpu_ctx* ctx) {
+  delete ctx;
+  return FCT_OK;
+}
+
+FCT_API const char* fct_wfbp_gpu_last_error(const fct_wfbp_gpu_ctx* ctx) {
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* scan,
+                                     const fct_recon_config* recon, const fct_exec_options* opts,
+                                     void* stream, fct_wfbp_gpu_plan** out) {
+  if (!ctx || !out) return FCT_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int st = validate(ctx, scan, recon, opts);
+  if (st != FCT_OK) return st;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  auto* p = new (std::nothrow) fct_wfbp_gpu_plan();
+  if (!p) return FCT_ERR_OUT_OF_MEMORY;
+  p->ctx = ctx;
+  p->device = ctx->device;
+  p->scan = *scan;
+  p->recon = *recon;
+  if (opts) p->opts = *opts;
+  if (stream) {
+    p->stream = static_cast<hipStream_t>(stream);
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete p;
+      set_err(ctx, "hipStreamCreate failed: %s", hipGetErrorString(e));
+      return FCT_ERR_DEVICE;
+    }
+    p->own_stream = true;
+  }
+  auto fail = [&](int code) {
+    free_plan_buffers(p);
+    if (p->own_stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return code;
+  };
+#define PLAN_TRY(expr)                                                             \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_err(ctx, "%s failed: %s", #expr, hipGetErrorString(e_));                 \
+      return fail(e_ == hipErrorOutOfMemory ? FCT_ERR_OUT_OF_MEMORY : FCT_ERR_DEVICE); \
+    }                                                                              \
+  } while (0)
+
+  const fct_scan_geometry& s = *scan;
+  const fct_recon_config& r = *recon;
+  p->ncols = s.n_channels;
+  p->nrows = s.n_rows;
+  p->nviews = s.n_views;
+  p->ppr = s.views_per_rotation;
+  p->H = p->ppr / 2;
+  int n = 1, lg = 0;
+  while (n < 2 * p->ncols) {
+    n <<= 1;
+    ++lg;
+  }
+  p->nfft = n;                                    // ReconstructionMachine.cu:54
+  p->log2n = lg;
+  p->cpitch = (p->ncols + 63) / 64 * 64;
+  p->dgamma = s.channel_spacing / s.dsd;
+  p->ds = s.channel_spacing * s.r / s.dsd;        // ReconstructionMachine.cu:491
+  p->dtheta = 2.0 * kPi / (double)p->ppr;
+  p->dz = s.feed / (double)p->ppr;
+  p->csw = s.r * s.row_spacing / s.dsd;           // ReconstructionMachine.cu:62
+  p->acq = 2.0 * s.r * std::sin(0.5 * ((double)p->ncols - 1.0) * p->dgamma);
+  p->g = fct_tables::bp_consts(p->ncols, p->nrows, s.channel_spacing, s.row_spacing, s.dsd, s.r,
+                               s.central_channel, s.central_row, s.feed);
+  p->scale = (float)(2.0 * kPi / (double)p->ppr);
+  p->tile_z = (opts && opts->tile_z) ? opts->tile_z : 16;
+  p->variant = opts ? opts->bp_variant : 0;
+  if (p->nfft > 4096) return fail((set_err(ctx, "nfft > 4096 unsupported"), FCT_ERR_LIMIT));
+
+  // Slice grids (ReconstructionMachine.cu:107-137) in FP64.
+  const double pitch = r.slice_pitch > 0 ? r.slice_pitch : r.slice_thickness;
+  p->n_native = fct_tables::slice_count(r.start_pos, r.end_pos, p->csw);
+  p->n_req = fct_tables::slice_count(r.start_pos, r.end_pos, pitch);
+  if (p->n_native < 1 || p->n_req < 1) return fail((set_err(ctx, "empty slice grid"), FCT_ERR_INVALID_ARGUMENT));
+  std::vector<double> zn(p->n_native), zr(p->n_req);
+  for (int i = 0; i < p->n_native; ++i) zn[i] = fct_tables::slice_z(r.start_pos, r.end_pos, p->csw, i);
+  for (int i = 0; i < p->n_req; ++i) zr[i] = fct_tables::slice_z(r.start_pos, r.end_pos, pitch, i);
+  const int chunk = fct_tables::kSliceChunk;
+  p->nchunks = (p->n_native + chunk - 1) / chunk;
+  p->nslabs = (p->n_native + p->tile_z - 1) / p->tile_z;
+
+  std::vector<float> xs(r.nx), ys(r.ny), zs(p->n_native), zt((size_t)p->nchunks * p->nviews);
+  for (int i = 0; i < r.nx; ++i) xs[i] = fct_tables::voxel_coord(i, r.nx, r.recon_fov, r.x_origin);
+  for (int i = 0; i < r.ny; ++i) ys[i] = fct_tables::voxel_coord(i, r.ny, r.recon_fov, r.y_origin);
+  for (int i = 0; i < p->n_native; ++i) zs[i] = (float)(zn[i] - zn[(i / chunk) * chunk]);
+  for (int c = 0; c < p->nchunks; ++c)
+    for (int v = 0; v < p->nviews; ++v)
+      zt[(size_t)c * p->nviews + v] = fct_tables::view_z_rel(s.z0, p->dz, v, zn[c * chunk]);
+  std::vector<float2> cs(p->H);
+  for (int t = 0; t < p->H; ++t) fct_tables::view_cos_sin(s.angle0, p->dtheta, t, &cs[t].x, &cs[t].y);
+
+  // Views a slab can select: table z within zrot of a slice (plus margin).
+  std::vector<int2> vr(p->nslabs);
+  for (int sl = 0; sl < p->nslabs; ++sl) {
+    const int a0 = sl * p->tile_z, a1 = std::min(p->n_native, a0 + p->tile_z) - 1;
+    const double zlo = std::min(zn[a0], zn[a1]) - std::fabs(s.feed);
+    const double zhi = std::max(zn[a0], zn[a1]) + std::fabs(s.feed);
+    double va = (zlo - s.z0) / p->dz, vb = (zhi - s.z0) / p->dz;
+    if (va > vb) std::swap(va, vb);
+    const double lo = std::floor(va) - 2, hi = std::ceil(vb) + 3;
+    vr[sl].x = (int)std::max(0.0, std::min(lo, (double)p->nviews));
+    vr[sl].y = (int)std::max(0.0, std::min(hi, (double)p->nviews));
+  }
+
+  // Rebinning table (cuda_kernels.cuh:17-36).
+  std::vector<fct::RebinDev> tab(p->ncols);
+  for (int c = 0; c < p->ncols; ++c) {
+    const fct_tables::RebinEntry e =
+        fct_tables::rebin_entry(c, s.central_channel, p->dgamma, p->dtheta, p->ncols);
+    tab[c] = fct::RebinDev{e.b0, e.b1, e.aoff, e.valid, (float)e.fb, (float)e.fa};
+  }
+
+  // Filter spectrum (generate_filter, ReconstructionMachine.cu:478-544):
+  // taps -> fftshift -> DFT in FP64; the spectrum of the symmetric kernel is
+  // real.  1/nfft (cuda_kernels.cuh:56-57) is folded in.
+  const std::vector<double> h =
+      filter_taps(p->nfft, p->ds, r.filter_kind, r.filter_c, r.filter_a);
+  std::vector<std::complex<double>> Hc(p->nfft);
+  for (int k = 0; k < p->nfft; ++k) {
+    const int lag = k < p->nfft / 2 ? k : k - p->nfft;
+    Hc[k] = h[lag + p->nfft / 2];
+  }
+  fft64(Hc);
+  std::vector<float> Hn(p->nfft);
+  for (int k = 0; k < p->nfft; ++k) Hn[k] = (float)(Hc[k].real() / (double)p->nfft);
+  std::vector<float2> tw(p->nfft);
+  for (int m = 0; m < p->nfft; ++m) {
+    const double ang = -2.0 * kPi * (double)m / (double)p->nfft;
+    tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+
+  // Thickening table (cuda_kernels.cuh:298-316): weights normalised in FP64.
+  std::vector<int32_t> toff(p->n_req + 1, 0), tidx;
+  std::vector<float> tw8;
+  for (int so = 0; so < p->n_req; ++so) {
+    double wsum = 0.0;
+    std::vector<std::pair<int, double>> ws;
+    for (int i = 0; i < p->n_native; ++i) {
+      const double w = std::max(0.0, 1.0 - std::fabs(zn[i] - zr[so]) / r.slice_thickness);
+      if (w != 0.0) {
+        ws.emplace_back(i, w);
+        wsum += w;
+      }
+    }
+    for (auto& pr : ws) {
+      tidx.push_back(pr.first);
+      tw8.push_back((float)(pr.second / wsum));
+    }
+    toff[so + 1] = (int32_t)tidx.size();
+  }
+  if (tidx.empty()) {
+    tidx.push_back(0);
+    tw8.push_back(0.0f);
+  }
+
+  PLAN_TRY(upload(&p->d_xs, xs, p->stream));
+  PLAN_TRY(upload(&p->d_ys, ys, p->stream));
+  PLAN_TRY(upload(&p->d_cs, cs, p->stream));
+  PLAN_TRY(upload(&p->d_zs, zs, p->stream));
+  PLAN_TRY(upload(&p->d_zt, zt, p->stream));
+  PLAN_TRY(upload(&p->d_slab_vrange, vr, p->stream));
+  PLAN_TRY(upload(&p->d_rebin, tab, p->stream));
+  PLAN_TRY(upload(&p->d_Hn, Hn, p->stream));
+  PLAN_TRY(upload(&p->d_tw, tw, p->stream));
+  PLAN_TRY(upload(&p->d_thk_off, toff, p->stream));
+  PLAN_TRY(upload(&p->d_thk_idx, tidx, p->stream));
+  PLAN_TRY(upload(&p->d_thk_w, tw8, p->stream));
+#undef PLAN_TRY
+  *out = p;
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_plan_destroy(fct_wfbp_gpu_plan* p) {
+  if (!p) return FCT_OK;
+  (void)hipSetDevice(p->device);
+  (void)hipStreamSynchronize(p->stream);
+  free_plan_buffers(p);
+  if (p->own_stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_plan_info(const fct_wfbp_gpu_plan* p, fct_plan_info* info) {
+  if (!p || !info) return FCT_ERR_INVALID_ARGUMENT;
+  std::memset(info, 0, sizeof(*info));
+  info->n_slices_native = p->n_native;
+  info->n_slices_requested = p->n_req;
+  info->nfft = p->nfft;
+  info->channel_pitch = p->cpitch;
+  info->raw_elems = (int64_t)p->nviews * p->nrows * p->ncols;
+  info->filtered_bytes = (int64_t)p->filtered_bytes();
+  info->row_scale_elems = (int64_t)p->row_scale_elems();
+  info->native_elems = (int64_t)p->n_native * p->recon.ny * p->recon.nx;
+  info->volume_elems = (int64_t)p->n_req * p->recon.ny * p->recon.nx;
+  info->collimated_slice_width = p->csw;
+  info->acquisition_fov = p->acq;
+  info->z_rot = std::fabs(p->scan.feed);
+  info->tile_z = p->tile_z;
+  info->bp_variant = p->variant;
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_set_stream(fct_wfbp_gpu_plan* p, void* stream) {
+  if (!p) return FCT_ERR_INVALID_ARGUMENT;
+  if (p->own_stream) {
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipStreamDestroy(p->stream);
+    p->own_stream = false;
+  }
+  if (stream) {
+    p->stream = static_cast<hipStream_t>(stream);
+  } else {
+    HIP_TRY(p->ctx, hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    p->own_stream = true;
+  }
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, void* filtered,
+                                      float* row_scale) {
+  if (!p || !raw || !filtered) return FCT_ERR_INVALID_ARGUMENT;
+  if (p->opts.precision != FCT_PRECISION_FP32 && !row_scale)
+    return set_err(p->ctx, "row_scale is required for FP16/BF16"), FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  fct::RebinFilterArgs a{};
+  a.raw = raw;
+  a.out = filtered;
+  a.row_scale = row_scale;
+  a.tab = p->d_rebin;
+  a.Hn = p->d_Hn;
+  a.tw = p->d_tw;
+  a.nviews = p->nviews;
+  a.nrows = p->nrows;
+  a.ncols = p->ncols;
+  a.cpitch = p->cpitch;
+  a.nfft = p->nfft;
+  a.log2n = p->log2n;
+  a.precision = p->opts.precision;
+  a.nlines = p->nviews * p->nrows;
+  HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_backproject(fct_wfbp_gpu_plan* p, const void* filtered,
+                                     const float* row_scale, float* native) {
+  if (!p || !filtered || !native) return FCT_ERR_INVALID_ARGUMENT;
+  if (p->opts.precision != FCT_PRECISION_FP32 && !row_scale)
+    return set_err(p->ctx, "row_scale is required for FP16/BF16"), FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  fct::BpArgs a = bp_args(p);
+  a.filt = filtered;
+  a.row_scale = row_scale;
+  a.native = native;
+  HIP_TRY(p->ctx, fct::launch_backproject(a, p->tile_z, p->variant, p->opts.precision,
+                                          fct::kBpAccumulate, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* p, const float* native, float* volume) {
+  if (!p || !native || !volume) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  fct::ThickenArgs a{};
+  a.native = native;
+  a.out = volume;
+  a.off = p->d_thk_off;
+  a.idx = p->d_thk_idx;
+  a.w = p->d_thk_w;
+  a.plane = p->recon.nx * p->recon.ny;
+  a.n_req = p->n_req;
+  HIP_TRY(p->ctx, fct::launch_thicken(a, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* p, const float* raw, float* volume,
+                                     int32_t host_buffers) {
+  if (!p || !raw || !volume) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  const size_t raw_bytes = (size_t)p->nviews * p->nrows * p->ncols * sizeof(float);
+  const size_t nat_bytes = (size_t)p->n_native * p->recon.ny * p->recon.nx * sizeof(float);
+  const size_t vol_bytes = (size_t)p->n_req * p->recon.ny * p->recon.nx * sizeof(float);
+  if (!p->d_filtered) HIP_TRY(p->ctx, hipMalloc(&p->d_filtered, p->filtered_bytes()));
+  if (p->row_scale_elems() && !p->d_row_scale)
+    HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_row_scale),
+                              p->row_scale_elems() * sizeof(float)));
+  if (!p->d_native) HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_native), nat_bytes));
+  const float* d_raw = raw;
+  float* d_vol = volume;
+  if (host_buffers) {
+    if (!p->d_raw_stage) HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_raw_stage), raw_bytes));
+    if (!p->d_vol_stage) HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_vol_stage), vol_bytes));
+    HIP_TRY(p->ctx, hipMemcpyAsync(p->d_raw_stage, raw, raw_bytes, hipMemcpyHostToDevice, p->stream));
+    d_raw = p->d_raw_stage;
+    d_vol = p->d_vol_stage;
+  }
+  int st = fct_wfbp_gpu_rebin_filter(p, d_raw, p->d_filtered, p->d_row_scale);
+  if (st) return st;
+  st = fct_wfbp_gpu_backproject(p, p->d_filtered, p->d_row_scale, p->d_native);
+  if (st) return st;
+  st = fct_wfbp_gpu_thicken(p, p->d_native, d_vol);
+  if (st) return st;
+  if (host_buffers) {
+    HIP_TRY(p->ctx, hipMemcpyAsync(volume, d_vol, vol_bytes, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(p->ctx, hipStreamSynchronize(p->stream));
+  }
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_count_voxel_views(fct_wfbp_gpu_plan* p, int32_t* per_voxel,
+                                           int64_t* total) {
+  if (!p || !total) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  const size_t nat = (size_t)p->n_native * p->recon.ny * p->recon.nx;
+  int32_t* counts = per_voxel;
+  unsigned long long* d_tot = nullptr;
+  bool own_counts = false;
+  if (!counts) {
+    HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&counts), nat * sizeof(int32_t)));
+    own_counts = true;
+  }
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_tot), sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemsetAsync(d_tot, 0, sizeof(unsigned long long), p->stream);
+  fct::BpArgs a = bp_args(p);
+  a.counts = counts;
+  a.total = d_tot;
+  if (e == hipSuccess)
+    e = fct::launch_backproject(a, p->tile_z, p->variant, FCT_PRECISION_FP32, fct::kBpCount, p->stream);
+  unsigned long long tot = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&tot, d_tot, sizeof tot, hipMemcpyDeviceToHost, p->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  if (d_tot) (void)hipFree(d_tot);
+  if (own_counts) (void)hipFree(counts);
+  if (e != hipSuccess) {
+    set_err(p->ctx, "count_voxel_views: %s", hipGetErrorString(e));
+    return FCT_ERR_DEVICE;
+  }
+  *total = (int64_t)tot;
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_voxel_selection(fct_wfbp_gpu_plan* p, int32_t ix, int32_t iy,
+                                         int32_t islice, int32_t* sel) {
+  if (!p || !sel) return FCT_ERR_INVALID_ARGUMENT;
+  if (ix < 0 || iy < 0 || islice < 0 || ix >= p->recon.nx || iy >= p->recon.ny ||
+      islice >= p->n_native)
+    return set_err(p->ctx, "voxel outside the native grid"), FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  HIP_TRY(p->ctx, hipMemsetAsync(sel, 0xFF, (size_t)p->nviews * sizeof(int32_t), p->stream));
+  fct::BpArgs a = bp_args(p);
+  a.sel = sel;
+  a.sel_ix = ix;
+  a.sel_iy = iy;
+  a.sel_is = islice;
+  HIP_TRY(p->ctx, fct::launch_backproject(a, p->tile_z, p->variant, FCT_PRECISION_FP32,
+                                          fct::kBpSelect, p->stream));
+  HIP_TRY(p->ctx, hipStreamSynchronize(p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_synchronize(fct_wfbp_gpu_plan* p) {
+  if (!p) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  HIP_TRY(p->ctx, hipStreamSynchronize(p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_phantom_project(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* scan,
+                                         double cyl_radius, double cyl_mu,
+                                         const fct_ellipsoid* ellipsoids, int32_t n_ellipsoids,
+                                         double i0, uint64_t seed, int32_t v0, int32_t v1,
+                                         float* raw, void* stream) {
+  if (!ctx || !scan || !raw) return FCT_ERR_INVALID_ARGUMENT;
+  if (n_ellipsoids < 0 || (n_ellipsoids > 0 && !ellipsoids) || n_ellipsoids > 4096)
+    return set_err(ctx, "bad ellipsoid list"), FCT_ERR_INVALID_ARGUMENT;
+  if (v0 < 0 || v1 > scan->n_views || v0 >= v1 || v1 - v0 > 65535)
+    return set_err(ctx, "bad view range [%d, %d)", v0, v1), FCT_ERR_INVALID_ARGUMENT;
+  if (scan->views_per_rotation < 1 || scan->n_channels < 1 || scan->n_rows < 1 ||
+      scan->n_rows > 65535)
+    return set_err(ctx, "bad scan geometry"), FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  fct_ellipsoid* d_ell = nullptr;
+  if (n_ellipsoids > 0) {
+    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&d_ell), n_ellipsoids * sizeof(fct_ellipsoid)));
+    hipError_t e = hipMemcpyAsync(d_ell, ellipsoids, n_ellipsoids * sizeof(fct_ellipsoid),
+                                  hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      (void)hipFree(d_ell);
+      set_err(ctx, "ellipsoid upload: %s", hipGetErrorString(e));
+      return FCT_ERR_DEVICE;
+    }
+  }
+  fct::PhantomArgs a{};
+  a.out = raw;
+  a.ell = d_ell;
+  a.n_ell = n_ellipsoids;
+  a.ncols = scan->n_channels;
+  a.nrows = scan->n_rows;
+  a.v0 = v0;
+  a.v1 = v1;
+  a.r = scan->r;
+  a.dsd = scan->dsd;
+  a.dgamma = scan->channel_spacing / scan->dsd;
+  a.drow = scan->row_spacing;
+  a.cc = scan->central_channel;
+  a.cr = scan->central_row;
+  a.angle0 = scan->angle0;
+  a.dtheta = 2.0 * kPi / (double)scan->views_per_rotation;
+  a.z0 = scan->z0;
+  a.dz = scan->feed / (double)scan->views_per_rotation;
+  a.cyl_r = cyl_radius;
+  a.cyl_mu = cyl_mu;
+  a.i0 = i0;
+  a.seed = seed;
+  hipError_t e = fct::launch_phantom(a, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (d_ell) (void)hipFree(d_ell);
+  if (e != hipSuccess) {
+    set_err(ctx, "phantom_project: %s", hipGetErrorString(e));
+    return FCT_ERR_DEVICE;
+  }
+  return FCT_OK;
+}
+
+}  // extern "C"

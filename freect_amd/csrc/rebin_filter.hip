@@ -1,0 +1,175 @@
+// rebin_filter.hip -- fused fan->parallel rebinning + ramp filtering.
+//
+// Replaces, per detector row sheet, the reference's rebin_kernel (texture
+// bilinear, cuda_kernels.cuh:11-39), cuFFT R2C, multiply_filter
+// (cuda_kernels.cuh:41-58), cuFFT C2R and reshape_rebin_into_final_array
+// (cuda_kernels.cuh:60-77), launched from ReconstructionMachine.cu:241-292.
+//
+// One workgroup owns two filter lines (line = (parallel view j, row r)).  Both
+// are rebinned straight into an LDS complex buffer as real and imaginary
+// parts, transformed with one LDS-resident Stockham FFT, multiplied by the
+// real filter spectrum H/N and transformed back through conj(FFT(conj(.))).
+// Because H is real and even (the spatial kernel is symmetric), the real and
+// imaginary parts of the result are the two filtered lines.  The lines occupy
+// ncols <= N/2 samples, so the circular product equals the linear
+// convolution y[i] = sum_j h[i-j] x[j], exactly what the reference computes.
+//
+// Storage: FP32, or FP16/BF16 with a power-of-two scale per line chosen so
+// that max|y|/scale <= 2^15.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include "fft_lds.h"
+#include "wfbp_internal.h"
+
+namespace fct {
+namespace {
+
+constexpr int kRfThreads = 256;
+
+__device__ __forceinline__ float rebin_sample(const float* __restrict__ raw, int j, int row,
+                                              const RebinDev& e, int nviews, int nrows,
+                                              int ncols) {
+  // Bilinear at (fan view j + aoff + fa, fan channel b0 + fb) with the view
+  // index clamped to the scan, as the reference's clamp-addressed texture.
+  int va = j + e.aoff;
+  int vb = va + 1;
+  va = min(max(va, 0), nviews - 1);
+  vb = min(max(vb, 0), nviews - 1);
+  const float* ra = raw + ((size_t)va * nrows + row) * ncols;
+  const float* rb = raw + ((size_t)vb * nrows + row) * ncols;
+  const float a0 = ra[e.b0], a1 = ra[e.b1];
+  const float c0 = rb[e.b0], c1 = rb[e.b1];
+  const float lo = fmaf(e.fa, c0 - a0, a0);
+  const float hi = fmaf(e.fa, c1 - a1, a1);
+  return fmaf(e.fb, hi - lo, lo);
+}
+
+__device__ FctCplx* fft_forward(FctCplx* a, FctCplx* b, const FctCplx* __restrict__ tw, int N,
+                                int log2n) {
+  int Ns = 1, rem = log2n;
+  while (rem > 0) {
+    if (rem >= 2) {
+      for (int j = threadIdx.x; j < (N >> 2); j += kRfThreads) fct_fft_r4(a, b, tw, N, Ns, j);
+      Ns <<= 2;
+      rem -= 2;
+    } else {
+      for (int j = threadIdx.x; j < (N >> 1); j += kRfThreads) fct_fft_r2(a, b, tw, N, Ns, j);
+      Ns <<= 1;
+      rem -= 1;
+    }
+    __syncthreads();
+    FctCplx* t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(kRfThreads) void rebin_filter_kernel(RebinFilterArgs a) {
+  extern __shared__ FctCplx lds[];
+  __shared__ float red[2][kRfThreads / 64];
+  const int N = a.nfft;
+  FctCplx* buf0 = lds;
+  FctCplx* buf1 = lds + N;
+  const int l0 = 2 * blockIdx.x;
+  const int l1 = l0 + 1;
+  const bool has1 = l1 < a.nlines;
+  const int j0 = l0 / a.nrows, r0 = l0 - j0 * a.nrows;
+  const int j1 = has1 ? l1 / a.nrows : 0, r1 = has1 ? l1 - j1 * a.nrows : 0;
+
+  for (int c = threadIdx.x; c < N; c += kRfThreads) {
+    float va = 0.0f, vb = 0.0f;
+    if (c < a.ncols) {
+      const RebinDev e = a.tab[c];
+      if (e.valid) {   // fan channel outside [0, ncols): 0 (cuda_kernels.cuh:32-34)
+        va = rebin_sample(a.raw, j0, r0, e, a.nviews, a.nrows, a.ncols);
+        if (has1) vb = rebin_sample(a.raw, j1, r1, e, a.nviews, a.nrows, a.ncols);
+      }
+    }
+    buf0[c] = FctCplx{va, vb};
+  }
+  __syncthreads();
+  const FctCplx* tw = reinterpret_cast<const FctCplx*>(a.tw);
+  FctCplx* X = fft_forward(buf0, buf1, tw, N, a.log2n);
+  FctCplx* other = X == buf0 ? buf1 : buf0;
+  for (int c = threadIdx.x; c < N; c += kRfThreads) {
+    const float h = a.Hn[c];
+    const FctCplx z = X[c];
+    X[c] = FctCplx{z.x * h, -z.y * h};   // conj(H/N * Z)
+  }
+  __syncthreads();
+  FctCplx* Y = fft_forward(X, other, tw, N, a.log2n);   // y = conj(Y)
+
+  const size_t o0 = (size_t)l0 * a.cpitch, o1 = (size_t)l1 * a.cpitch;
+  if (a.precision == FCT_PRECISION_FP32) {
+    float* out = static_cast<float*>(a.out);
+    for (int c = threadIdx.x; c < a.cpitch; c += kRfThreads) {
+      const bool in = c < a.ncols;
+      out[o0 + c] = in ? Y[c].x : 0.0f;
+      if (has1) out[o1 + c] = in ? -Y[c].y : 0.0f;
+    }
+    return;
+  }
+  // Per-line power-of-two scale: max|y| / scale <= 2^15.
+  float m0 = 0.0f, m1 = 0.0f;
+  for (int c = threadIdx.x; c < a.ncols; c += kRfThreads) {
+    m0 = fmaxf(m0, fabsf(Y[c].x));
+    m1 = fmaxf(m1, fabsf(Y[c].y));
+  }
+  m0 = wave_max(m0);
+  m1 = wave_max(m1);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = m0;
+    red[1][w] = m1;
+  }
+  __syncthreads();
+  m0 = red[0][0];
+  m1 = red[1][0];
+  for (int i = 1; i < kRfThreads / 64; ++i) {
+    m0 = fmaxf(m0, red[0][i]);
+    m1 = fmaxf(m1, red[1][i]);
+  }
+  int e0 = 0, e1 = 0;
+  frexpf(m0, &e0);
+  frexpf(m1, &e1);
+  const float s0 = m0 > 0.0f ? ldexpf(1.0f, e0 - 15) : 1.0f;
+  const float s1 = m1 > 0.0f ? ldexpf(1.0f, e1 - 15) : 1.0f;
+  const float i0 = 1.0f / s0, i1 = 1.0f / s1;   // exact: powers of two
+  if (threadIdx.x == 0) {
+    a.row_scale[l0] = s0;
+    if (has1) a.row_scale[l1] = s1;
+  }
+  if (a.precision == FCT_PRECISION_FP16) {
+    __half* out = static_cast<__half*>(a.out);
+    for (int c = threadIdx.x; c < a.cpitch; c += kRfThreads) {
+      const bool in = c < a.ncols;
+      out[o0 + c] = __float2half_rn(in ? Y[c].x * i0 : 0.0f);
+      if (has1) out[o1 + c] = __float2half_rn(in ? -Y[c].y * i1 : 0.0f);
+    }
+  } else {
+    __hip_bfloat16* out = static_cast<__hip_bfloat16*>(a.out);
+    for (int c = threadIdx.x; c < a.cpitch; c += kRfThreads) {
+      const bool in = c < a.ncols;
+      out[o0 + c] = __float2bfloat16(in ? Y[c].x * i0 : 0.0f);
+      if (has1) out[o1 + c] = __float2bfloat16(in ? -Y[c].y * i1 : 0.0f);
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s) {
+  const int blocks = (a.nlines + 1) / 2;
+  const size_t lds = 2 * (size_t)a.nfft * sizeof(FctCplx);
+  hipLaunchKernelGGL(rebin_filter_kernel, dim3(blocks), dim3(kRfThreads), lds, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace fct

@@ -1,0 +1,133 @@
+// wfbp_internal.h -- plan layout and kernel launcher interface (not exported).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fct_wfbp_gpu.h"
+#include "wfbp_math.h"
+
+namespace fct {
+
+// Device-side rebinning entry per parallel channel (FP32 weights from the FP64
+// table recipe of wfbp_math.h).
+struct RebinDev {
+  int32_t b0, b1, aoff, valid;
+  float fb, fa;
+};
+
+// Everything the rebin + filter kernel reads.
+struct RebinFilterArgs {
+  const float* raw;          // [view][row][ncols]
+  void* out;                 // [view][row][cpitch]
+  float* row_scale;          // [view][row] or null
+  const RebinDev* tab;       // [ncols]
+  const float* Hn;           // [nfft] real filter spectrum / nfft
+  const float2* tw;          // [nfft] exp(-2 pi i k / nfft)
+  int32_t nviews, nrows, ncols, cpitch, nfft, log2n;
+  int32_t precision;
+  int32_t nlines;            // nviews * nrows
+};
+
+// Everything the backprojection kernel reads.
+struct BpArgs {
+  const void* filt;          // [view][row][cpitch] FP32 / FP16 / BF16
+  const float* row_scale;    // [view][row] (FP16/BF16 only)
+  float* native;             // [slice][y][x]
+  int32_t* counts;           // count mode: [slice][y][x]
+  unsigned long long* total; // count mode
+  int32_t* sel;              // selection mode: [nviews]
+  const float* xs;           // [nx]
+  const float* ys;           // [ny]
+  const float2* cs;          // [H] (cos, sin)
+  const float* zs;           // [n_native] slice z relative to its chunk reference
+  const float* zt;           // [nchunks][nviews] view z relative to the chunk reference
+  const int2* slab_vrange;   // [nslabs] views [x, y) a slab can select
+  FctBpConsts g;
+  float scale;               // 2 pi / ppr
+  int32_t nx, ny, n_native, nviews, H, cpitch;
+  int32_t tiles_x, tiles_y, nslabs;
+  int32_t sel_ix, sel_iy, sel_is;
+};
+
+struct ThickenArgs {
+  const float* native;
+  float* out;
+  const int32_t* off;        // [n_req + 1] into idx/w
+  const int32_t* idx;        // native slice indices
+  const float* w;            // normalised weights
+  int32_t plane;             // nx*ny
+  int32_t n_req;
+};
+
+struct PhantomArgs {
+  float* out;
+  const fct_ellipsoid* ell;  // device copy
+  int32_t n_ell;
+  int32_t ncols, nrows, v0, v1;
+  double r, dsd, dgamma, drow, cc, cr, angle0, dtheta, z0, dz;
+  double cyl_r, cyl_mu, i0;
+  unsigned long long seed;
+};
+
+hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s);
+hipError_t launch_backproject(const BpArgs& a, int tile_z, int variant, int precision, int mode,
+                              hipStream_t s);
+hipError_t launch_thicken(const ThickenArgs& a, hipStream_t s);
+hipError_t launch_phantom(const PhantomArgs& a, hipStream_t s);
+
+enum BpMode { kBpAccumulate = 0, kBpCount = 1, kBpSelect = 2 };
+
+}  // namespace fct
+
+struct fct_wfbp_gpu_ctx {
+  int device = 0;
+  std::string last_error;
+};
+
+struct fct_wfbp_gpu_plan {
+  fct_wfbp_gpu_ctx* ctx = nullptr;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  fct_scan_geometry scan{};
+  fct_recon_config recon{};
+  fct_exec_options opts{};
+
+  int ncols = 0, nrows = 0, nviews = 0, ppr = 0, H = 0, nfft = 0, log2n = 0, cpitch = 0;
+  int n_native = 0, n_req = 0, nchunks = 0, tile_z = 16, variant = 0, nslabs = 0;
+  double dgamma = 0, ds = 0, dtheta = 0, dz = 0, csw = 0, acq = 0;
+  FctBpConsts g{};
+  float scale = 0;
+
+  // Device tables owned by the plan.
+  float* d_xs = nullptr;
+  float* d_ys = nullptr;
+  float2* d_cs = nullptr;
+  float* d_zs = nullptr;
+  float* d_zt = nullptr;
+  int2* d_slab_vrange = nullptr;
+  fct::RebinDev* d_rebin = nullptr;
+  float* d_Hn = nullptr;
+  float2* d_tw = nullptr;
+  int32_t* d_thk_off = nullptr;
+  int32_t* d_thk_idx = nullptr;
+  float* d_thk_w = nullptr;
+
+  // Intermediate buffers for fct_wfbp_gpu_reconstruct (allocated on first use).
+  void* d_filtered = nullptr;
+  float* d_row_scale = nullptr;
+  float* d_native = nullptr;
+  float* d_raw_stage = nullptr;
+  float* d_vol_stage = nullptr;
+
+  size_t filtered_bytes() const {
+    const size_t el = opts.precision == FCT_PRECISION_FP32 ? 4 : 2;
+    return (size_t)nviews * nrows * cpitch * el;
+  }
+  size_t row_scale_elems() const {
+    return opts.precision == FCT_PRECISION_FP32 ? 0 : (size_t)nviews * nrows;
+  }
+};

@@ -165,7 +165,6 @@ FCT_HD bool fct_bp_select(float q, float qidx, float pidx, const FctBpConsts& g,
 // ---------------------------------------------------------------------------
 // Host-only FP64 -> FP32 table recipes (used by the planner and the oracle).
 // ---------------------------------------------------------------------------
-#if !defined(__HIP_DEVICE_COMPILE__)
 #ifdef __cplusplus
 #include <cmath>
 namespace fct_tables {
@@ -269,5 +268,4 @@ inline double slice_z(double start, double end, double step, int i) {
 constexpr int kSliceChunk = 64;
 
 }  // namespace fct_tables
-#endif
 #endif

@@ -1,0 +1,291 @@
+"""Python mirror of the fct_wfbp_gpu C ABI (ctypes).
+
+This is the host-side mirror of FreeCT's ReconstructionMachine stages
+(ReconstructionMachine.h:16-21) for the test harness and the benchmark:
+``Plan.rebin_filter``, ``Plan.backproject``, ``Plan.thicken`` and
+``Plan.reconstruct`` call the HIP kernels through the C ABI declared in
+include/fct_wfbp_gpu.h.  PyTorch only provides device memory and the stream.
+There is no CPU fallback: a missing library raises at import of a plan.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import scans as S
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libfct_wfbp_gpu.so")
+
+PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+
+class FctError(RuntimeError):
+    pass
+
+
+class ScanGeometryC(C.Structure):
+    _fields_ = [("n_channels", C.c_int32), ("n_rows", C.c_int32), ("n_views", C.c_int32),
+                ("views_per_rotation", C.c_int32), ("channel_spacing", C.c_double),
+                ("row_spacing", C.c_double), ("dsd", C.c_double), ("r", C.c_double),
+                ("central_channel", C.c_double), ("central_row", C.c_double),
+                ("angle0", C.c_double), ("z0", C.c_double), ("feed", C.c_double)]
+
+
+class ReconConfigC(C.Structure):
+    _fields_ = [("start_pos", C.c_double), ("end_pos", C.c_double), ("recon_fov", C.c_double),
+                ("slice_thickness", C.c_double), ("slice_pitch", C.c_double),
+                ("nx", C.c_int32), ("ny", C.c_int32), ("x_origin", C.c_double),
+                ("y_origin", C.c_double), ("recon_kernel", C.c_int32),
+                ("filter_kind", C.c_int32), ("filter_c", C.c_double), ("filter_a", C.c_double),
+                ("tube_angle_offset", C.c_double), ("adaptive_filtration_s", C.c_double)]
+
+
+class ExecOptionsC(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("tile_z", C.c_int32), ("bp_variant", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class PlanInfoC(C.Structure):
+    _fields_ = [("n_slices_native", C.c_int32), ("n_slices_requested", C.c_int32),
+                ("nfft", C.c_int32), ("channel_pitch", C.c_int32), ("raw_elems", C.c_int64),
+                ("filtered_bytes", C.c_int64), ("row_scale_elems", C.c_int64),
+                ("native_elems", C.c_int64), ("volume_elems", C.c_int64),
+                ("collimated_slice_width", C.c_double), ("acquisition_fov", C.c_double),
+                ("z_rot", C.c_double), ("tile_z", C.c_int32), ("bp_variant", C.c_int32)]
+
+
+class EllipsoidC(C.Structure):
+    _fields_ = [("c", C.c_double * 3), ("a", C.c_double * 3), ("rot", C.c_double * 9),
+                ("dmu", C.c_double)]
+
+
+# Every exported entry point and its ctypes signature.
+_P, _VP = C.POINTER, C.c_void_p
+SIGNATURES = {
+    "fct_wfbp_gpu_create": (C.c_int, [C.c_int32, _P(_VP)]),
+    "fct_wfbp_gpu_destroy": (C.c_int, [_VP]),
+    "fct_wfbp_gpu_last_error": (C.c_char_p, [_VP]),
+    "fct_wfbp_gpu_plan_create": (C.c_int, [_VP, _P(ScanGeometryC), _P(ReconConfigC),
+                                           _P(ExecOptionsC), _VP, _P(_VP)]),
+    "fct_wfbp_gpu_plan_destroy": (C.c_int, [_VP]),
+    "fct_wfbp_gpu_plan_info": (C.c_int, [_VP, _P(PlanInfoC)]),
+    "fct_wfbp_gpu_set_stream": (C.c_int, [_VP, _VP]),
+    "fct_wfbp_gpu_rebin_filter": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "fct_wfbp_gpu_backproject": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "fct_wfbp_gpu_thicken": (C.c_int, [_VP, _VP, _VP]),
+    "fct_wfbp_gpu_reconstruct": (C.c_int, [_VP, _VP, _VP, C.c_int32]),
+    "fct_wfbp_gpu_count_voxel_views": (C.c_int, [_VP, _VP, _P(C.c_int64)]),
+    "fct_wfbp_gpu_voxel_selection": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _VP]),
+    "fct_wfbp_gpu_synchronize": (C.c_int, [_VP]),
+    "fct_wfbp_gpu_phantom_project": (C.c_int, [_VP, _P(ScanGeometryC), C.c_double, C.c_double,
+                                               _P(EllipsoidC), C.c_int32, C.c_double,
+                                               C.c_uint64, C.c_int32, C.c_int32, _VP, _VP]),
+}
+
+_lib = None
+
+
+def load_library():
+    """Load libfct_wfbp_gpu.so; raises FctError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FctError(f"{LIB_PATH} is missing: run __graft_entry__.build() first "
+                           "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def scan_struct(scan: S.ScanGeometry) -> ScanGeometryC:
+    return ScanGeometryC(scan.n_channels, scan.n_rows, scan.n_views, scan.views_per_rotation,
+                         scan.channel_spacing, scan.row_spacing, scan.dsd, scan.r,
+                         scan.central_channel, scan.central_row, scan.angle0, scan.z0, scan.feed)
+
+
+def recon_struct(rc: S.ReconConfig) -> ReconConfigC:
+    return ReconConfigC(rc.start_pos, rc.end_pos, rc.recon_fov, rc.slice_thickness,
+                        rc.slice_pitch, rc.nx, rc.ny, rc.x_origin, rc.y_origin, 0,
+                        rc.filter_kind, rc.filter_c, rc.filter_a, 0.0, 0.0)
+
+
+def _ptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _stream_handle(stream=None) -> int:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        L = load_library()
+        h = C.c_void_p()
+        st = L.fct_wfbp_gpu_create(device, C.byref(h))
+        if st != 0:
+            raise FctError(f"fct_wfbp_gpu_create({device}) failed with status {st}")
+        self.h = h
+        self.device = device
+
+    def check(self, st: int, what: str):
+        if st != 0:
+            msg = load_library().fct_wfbp_gpu_last_error(self.h)
+            raise FctError(f"{what}: status {st}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            load_library().fct_wfbp_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def phantom_project(self, scan: S.ScanGeometry, phantom: S.Phantom, i0: float = 0.0,
+                        seed: int = 0, out=None, v0: int = 0, v1: Optional[int] = None,
+                        stream=None):
+        """Seeded synthetic scan on the GPU: [view][row][ch] float32 tensor."""
+        import torch
+        v1 = scan.n_views if v1 is None else v1
+        if out is None:
+            out = torch.empty((v1 - v0, scan.n_rows, scan.n_channels), dtype=torch.float32,
+                              device=f"cuda:{self.device}")
+        n = len(phantom.ellipsoids)
+        arr = (EllipsoidC * max(1, n))()
+        for i, e in enumerate(phantom.ellipsoids):
+            arr[i].c[:] = list(e["c"])
+            arr[i].a[:] = list(e["a"])
+            arr[i].rot[:] = list(np.asarray(e["rot"], np.float64).reshape(9))
+            arr[i].dmu = e["dmu"]
+        sc = scan_struct(scan)
+        st = load_library().fct_wfbp_gpu_phantom_project(
+            self.h, C.byref(sc), phantom.cyl_radius, phantom.cyl_mu, arr, n, float(i0),
+            C.c_uint64(seed & ((1 << 64) - 1)), v0, v1, out.data_ptr(), _stream_handle(stream))
+        self.check(st, "phantom_project")
+        return out
+
+
+class Plan:
+    """One reconstruction setup (geometry + recon config + execution options)."""
+
+    def __init__(self, ctx: Context, scan: S.ScanGeometry, recon: S.ReconConfig,
+                 precision: str = "fp32", tile_z: int = 0, bp_variant: int = 0, stream=None):
+        self.ctx = ctx
+        self.scan = scan
+        self.recon = recon
+        self.precision = precision
+        sc, rc = scan_struct(scan), recon_struct(recon)
+        opts = ExecOptionsC(PRECISION[precision], tile_z, bp_variant, 0)
+        h = C.c_void_p()
+        st = load_library().fct_wfbp_gpu_plan_create(ctx.h, C.byref(sc), C.byref(rc),
+                                                     C.byref(opts), _stream_handle(stream),
+                                                     C.byref(h))
+        ctx.check(st, "plan_create")
+        self.h = h
+        info = PlanInfoC()
+        ctx.check(load_library().fct_wfbp_gpu_plan_info(self.h, C.byref(info)), "plan_info")
+        self.info = info
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().fct_wfbp_gpu_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- buffers ------------------------------------------------------------
+    def _dev(self):
+        return f"cuda:{self.ctx.device}"
+
+    def alloc_filtered(self):
+        import torch
+        i = self.info
+        filt = torch.empty(i.filtered_bytes, dtype=torch.uint8, device=self._dev())
+        scale = (torch.empty(i.row_scale_elems, dtype=torch.float32, device=self._dev())
+                 if i.row_scale_elems else None)
+        return filt, scale
+
+    def alloc_native(self):
+        import torch
+        i = self.info
+        return torch.empty((i.n_slices_native, self.recon.ny, self.recon.nx),
+                           dtype=torch.float32, device=self._dev())
+
+    def alloc_volume(self):
+        import torch
+        i = self.info
+        return torch.empty((i.n_slices_requested, self.recon.ny, self.recon.nx),
+                           dtype=torch.float32, device=self._dev())
+
+    def filtered_view(self, filt):
+        """The filtered buffer as a [view][row][channel_pitch] tensor."""
+        import torch
+        dt = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}[self.precision]
+        return filt.view(dt).view(self.scan.n_views, self.scan.n_rows, self.info.channel_pitch)
+
+    # -- stages -------------------------------------------------------------
+    def set_stream(self, stream=None):
+        self.ctx.check(load_library().fct_wfbp_gpu_set_stream(self.h, _stream_handle(stream)),
+                       "set_stream")
+
+    def rebin_filter(self, raw, filt, scale=None):
+        self.ctx.check(load_library().fct_wfbp_gpu_rebin_filter(self.h, _ptr(raw), _ptr(filt),
+                                                                _ptr(scale)), "rebin_filter")
+
+    def backproject(self, filt, scale, native):
+        self.ctx.check(load_library().fct_wfbp_gpu_backproject(self.h, _ptr(filt), _ptr(scale),
+                                                              _ptr(native)), "backproject")
+
+    def thicken(self, native, volume):
+        self.ctx.check(load_library().fct_wfbp_gpu_thicken(self.h, _ptr(native), _ptr(volume)),
+                       "thicken")
+
+    def reconstruct(self, raw, volume, host_buffers: bool = False):
+        self.ctx.check(load_library().fct_wfbp_gpu_reconstruct(self.h, _ptr(raw), _ptr(volume),
+                                                              1 if host_buffers else 0),
+                       "reconstruct")
+
+    def reconstruct_host(self, raw: np.ndarray) -> np.ndarray:
+        raw = np.ascontiguousarray(raw, np.float32)
+        vol = np.empty((self.info.n_slices_requested, self.recon.ny, self.recon.nx), np.float32)
+        self.ctx.check(load_library().fct_wfbp_gpu_reconstruct(
+            self.h, raw.ctypes.data, vol.ctypes.data, 1), "reconstruct(host)")
+        return vol
+
+    def count_voxel_views(self, per_voxel=None) -> int:
+        tot = C.c_int64(0)
+        self.ctx.check(load_library().fct_wfbp_gpu_count_voxel_views(self.h, _ptr(per_voxel),
+                                                                    C.byref(tot)),
+                       "count_voxel_views")
+        return tot.value
+
+    def voxel_selection(self, ix, iy, islice):
+        import torch
+        sel = torch.empty(self.scan.n_views, dtype=torch.int32, device=self._dev())
+        self.ctx.check(load_library().fct_wfbp_gpu_voxel_selection(self.h, ix, iy, islice,
+                                                                  sel.data_ptr()),
+                       "voxel_selection")
+        return sel
+
+    def synchronize(self):
+        self.ctx.check(load_library().fct_wfbp_gpu_synchronize(self.h), "synchronize")

@@ -1,0 +1,185 @@
+/* fct_wfbp_gpu.h -- C ABI of the MI355X-native WFBP engine.
+ *
+ * Drop-in boundary for FreeCT's fct_wfbp reconstruction path.  The reference
+ * has no FFI; its boundary is the recon YAML (parse_config.cpp:31-62), the
+ * fct::RawDataSet getters (RawDataSet.h:91-139), the raw float32 output volume
+ * (ReconstructionMachine.cu:436-455) and the fct::ReconstructionMachine
+ * methods (ReconstructionMachine.h:16-21).  Every entry point here takes plain
+ * pointers and sizes, returns an fct_status, never exits the process and never
+ * throws.  Device buffers are hipMalloc'd by the caller unless stated.
+ *
+ * Threading: a plan is bound to one device and one stream and is not
+ * re-entrant; distinct plans may be driven from distinct host threads.
+ */
+#ifndef FCT_WFBP_GPU_H
+#define FCT_WFBP_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCT_API __attribute__((visibility("default")))
+
+typedef enum {
+  FCT_OK = 0,
+  FCT_ERR_INVALID_ARGUMENT = 1,
+  FCT_ERR_UNSUPPORTED = 2,
+  FCT_ERR_OUT_OF_MEMORY = 3,
+  FCT_ERR_DEVICE = 4,
+  FCT_ERR_LIMIT = 5
+} fct_status;
+
+/* Precision of the filtered projections handed from rebin_filter to
+ * backproject.  Accumulation is FP32 in every mode. */
+typedef enum {
+  FCT_PRECISION_FP32 = 0,
+  FCT_PRECISION_FP16 = 1,   /* FP16 values x per-(view,row) FP32 scale */
+  FCT_PRECISION_BF16 = 2    /* BF16 values x per-(view,row) FP32 scale */
+} fct_precision;
+
+/* Scan description: what RawDataSet.h:91-121 exposes, with the per-view tube
+ * angle and table position as the linear law angle0 + v*2pi/ppr and
+ * z0 + v*feed/ppr (the reference's rebinning already assumes uniform views,
+ * cuda_kernels.cuh:15-19). */
+typedef struct {
+  int32_t n_channels;          /* getDetectorChannels */
+  int32_t n_rows;              /* getDetectorRows */
+  int32_t n_views;             /* getTotalNumProjections */
+  int32_t views_per_rotation;  /* getProjectionsPerRotation (must be even) */
+  double channel_spacing;      /* getDetectorTransverseSpacing, mm at detector */
+  double row_spacing;          /* getDetectorAxialSpacing, mm at detector */
+  double dsd;                  /* getDistSourceToDetector */
+  double r;                    /* getDistSourceToIsocenter */
+  double central_channel;      /* getDetectorCentralChannel */
+  double central_row;          /* getDetectorCentralRow */
+  double angle0;               /* getTubeAngle(0) */
+  double z0;                   /* getTablePosition(0) */
+  double feed;                 /* getTablePosition(ppr) - getTablePosition(0) */
+} fct_scan_geometry;
+
+/* Recon configuration: the keys of parse_config.cpp:39-57 plus filter
+ * selection.  recon_kernel, tube_angle_offset and adaptive_filtration_s are
+ * carried but unused, as in the reference (parse_config.cpp:53,56,57). */
+typedef struct {
+  double start_pos, end_pos;   /* mm, same frame as the table positions */
+  double recon_fov;            /* mm */
+  double slice_thickness;      /* mm */
+  double slice_pitch;          /* mm; 0 -> slice_thickness */
+  int32_t nx, ny;
+  double x_origin, y_origin;   /* mm */
+  int32_t recon_kernel;
+  int32_t filter_kind;         /* 0: generalised-Hamming ramp (c, a); 1: Shepp-Logan */
+  double filter_c, filter_a;   /* ramp cutoff and Hamming a (ramp.m:7-8) */
+  double tube_angle_offset;
+  double adaptive_filtration_s;
+} fct_recon_config;
+
+typedef struct {
+  int32_t precision;           /* fct_precision */
+  int32_t tile_z;              /* slices per backprojection tile: 0 = tuned default, or 8/16/32 */
+  int32_t bp_variant;          /* 0 = tuned default; see fct_wfbp_gpu_plan_info */
+  int32_t reserved;
+} fct_exec_options;
+
+typedef struct {
+  int32_t n_slices_native;     /* slices at the collimated width (ReconstructionMachine.cu:110-137) */
+  int32_t n_slices_requested;  /* slices at slice_pitch */
+  int32_t nfft;                /* 2^ceil(log2(2*ncols)) (ReconstructionMachine.cu:54) */
+  int32_t channel_pitch;       /* padded channels per filtered row */
+  int64_t raw_elems;           /* n_views*n_rows*n_channels floats, [view][row][ch] */
+  int64_t filtered_bytes;      /* bytes of the filtered buffer, [view][row][channel_pitch] */
+  int64_t row_scale_elems;     /* floats of the per-(view,row) scale buffer (0 for FP32) */
+  int64_t native_elems;        /* n_slices_native*ny*nx floats, [slice][y][x] */
+  int64_t volume_elems;        /* n_slices_requested*ny*nx floats, [slice][y][x] */
+  double collimated_slice_width;
+  double acquisition_fov;
+  double z_rot;
+  int32_t tile_z;
+  int32_t bp_variant;
+} fct_plan_info;
+
+typedef struct {
+  double c[3];                 /* centre, mm */
+  double a[3];                 /* semi-axes, mm */
+  double rot[9];               /* row-major rotation; body axis i = column i */
+  double dmu;                  /* additive attenuation, 1/mm */
+} fct_ellipsoid;
+
+typedef struct fct_wfbp_gpu_ctx fct_wfbp_gpu_ctx;
+typedef struct fct_wfbp_gpu_plan fct_wfbp_gpu_plan;
+
+/* Device selection and validation; replaces validate_selected_device
+ * (util.cu:6-35) and main.cpp's -d option (main.cpp:43-44). */
+FCT_API int fct_wfbp_gpu_create(int32_t device, fct_wfbp_gpu_ctx** ctx);
+FCT_API int fct_wfbp_gpu_destroy(fct_wfbp_gpu_ctx* ctx);
+/* Last error message of the context (the reference prints and exits,
+ * gpu_error_check.h:38-58). */
+FCT_API const char* fct_wfbp_gpu_last_error(const fct_wfbp_gpu_ctx* ctx);
+
+/* Setup: ConfigureCTGeometry + AllocateKeyArrays + generate_filter
+ * (ReconstructionMachine.cu:44-139, 478-544, recon_structs.h:69-83).
+ * FP64 view tables and slice grids, FP32 per-chunk offsets, filter spectrum,
+ * rebinning tables.  stream may be NULL (the plan creates its own). */
+FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* scan,
+                                     const fct_recon_config* recon,
+                                     const fct_exec_options* opts, void* stream,
+                                     fct_wfbp_gpu_plan** plan);
+FCT_API int fct_wfbp_gpu_plan_destroy(fct_wfbp_gpu_plan* plan);
+FCT_API int fct_wfbp_gpu_plan_info(const fct_wfbp_gpu_plan* plan, fct_plan_info* info);
+FCT_API int fct_wfbp_gpu_set_stream(fct_wfbp_gpu_plan* plan, void* stream);
+
+/* Rebin + ramp filter (ReconstructionMachine.cu:141-306, cuda_kernels.cuh:11-77).
+ * raw: device [view][row][ch] float32 fan data.  filtered: device buffer of
+ * filtered_bytes; row_scale: device buffer of row_scale_elems floats (NULL for
+ * FP32).  Asynchronous on the plan's stream. */
+FCT_API int fct_wfbp_gpu_rebin_filter(fct_wfbp_gpu_plan* plan, const float* raw,
+                                      void* filtered, float* row_scale);
+
+/* Helical-weighted backprojection into native slices
+ * (ReconstructionMachine.cu:308-385, cuda_kernels.cuh:79-93, 198-287). */
+FCT_API int fct_wfbp_gpu_backproject(fct_wfbp_gpu_plan* plan, const void* filtered,
+                                     const float* row_scale, float* native);
+
+/* Triangular slice thickening (ReconstructionMachine.cu:387-434,
+ * cuda_kernels.cuh:289-320). */
+FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* plan, const float* native, float* volume);
+
+/* Whole chain (RunReconstruction, ReconstructionMachine.cu:76-82).  raw and
+ * volume are device pointers unless host_buffers != 0, in which case they are
+ * host pointers and the call returns after the volume is copied back.  The
+ * plan owns the intermediate buffers. */
+FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* plan, const float* raw, float* volume,
+                                     int32_t host_buffers);
+
+/* Exact count of selected (voxel, view) samples, the throughput unit.
+ * per_voxel (optional, device, native_elems int32) receives the count of
+ * every native voxel; *total the sum. Synchronous. */
+FCT_API int fct_wfbp_gpu_count_voxel_views(fct_wfbp_gpu_plan* plan, int32_t* per_voxel,
+                                           int64_t* total);
+
+/* Selection of one native voxel for every parallel view: qf*65536+pf or -1
+ * (device buffer of n_views int32).  Synchronous.  Test hook for bit-exact
+ * index selection. */
+FCT_API int fct_wfbp_gpu_voxel_selection(fct_wfbp_gpu_plan* plan, int32_t ix, int32_t iy,
+                                         int32_t islice, int32_t* sel);
+
+FCT_API int fct_wfbp_gpu_synchronize(fct_wfbp_gpu_plan* plan);
+
+/* Seeded synthetic scan (no reference counterpart): analytic FP64 line
+ * integrals of a water cylinder plus ellipsoids through the fan geometry, and,
+ * when i0 > 0, Poisson noise N ~ P(i0 exp(-p)) with p = ln(i0 / max(N, 1/2)).
+ * ellipsoids is a HOST array.  raw: device [view][row][ch] float32.  Views
+ * [v0, v1) are written at raw[(v - v0)]. */
+FCT_API int fct_wfbp_gpu_phantom_project(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* scan,
+                                         double cyl_radius, double cyl_mu,
+                                         const fct_ellipsoid* ellipsoids, int32_t n_ellipsoids,
+                                         double i0, uint64_t seed, int32_t v0, int32_t v1,
+                                         float* raw, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FCT_WFBP_GPU_H */

@@ -1,0 +1,189 @@
+"""GPU parity of the HIP path against the FP64 CPU restatement (oracle/).
+
+Everything runs through the C ABI (include/fct_wfbp_gpu.h) via ctypes.
+Tolerances: index selection bit-exact; FP32 GPU volumes within 0.5 HU RMSE
+and 2 HU max of the FP64 oracle; filtered projections within 2e-6 of the
+row maximum (FP32 FFT vs FP64 direct sums).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from freect_amd import scans as S
+from helpers import fov_mask, gpu_ctx, hu_errors, say, to_np
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return gpu_ctx()
+
+
+def _plan(ctx, w, **kw):
+    from freect_amd import wfbp
+    return wfbp.Plan(ctx, w.scan, w.recon, **kw)
+
+
+def _raw_gpu(ctx, w, i0=None):
+    return ctx.phantom_project(w.scan, w.phantom, w.i0 if i0 is None else i0, w.seed)
+
+
+def test_phantom_matches_oracle(ctx, oracle):
+    w = S.tiny()
+    say("phantom tiny", w.scan.n_views, w.scan.n_rows, w.scan.n_channels)
+    raw = to_np(_raw_gpu(ctx, w, 0.0))
+    ref = oracle.project_phantom(w.scan, w.phantom)
+    assert np.max(np.abs(raw - ref)) <= 4 * np.spacing(np.float32(np.abs(ref).max()))
+
+
+def test_phantom_noise_statistics(ctx):
+    # Uniform line integral p: the noisy values average to ~p with variance
+    # ~ exp(p)/I0 (delta method on ln of a Poisson count).
+    w = S.tiny(n_channels=128, n_rows=8, ppr=128, rotations=4)
+    ph = S.Phantom(cyl_radius=1000.0, cyl_mu=0.0, ellipsoids=[])
+    clean = to_np(ctx.phantom_project(w.scan, ph, 0.0, 1))
+    assert np.abs(clean).max() == 0.0
+    i0 = 1e4
+    noisy = to_np(ctx.phantom_project(w.scan, ph, i0, 123))
+    assert abs(noisy.mean()) < 3 * math.sqrt(1.0 / i0 / noisy.size) + 1e-4
+    assert abs(noisy.var() * i0 - 1.0) < 0.05
+    again = to_np(ctx.phantom_project(w.scan, ph, i0, 123))
+    assert np.array_equal(noisy, again)  # seeded: deterministic
+
+
+@pytest.mark.parametrize("filt", ["ramp", "shepp-logan", "hann", "medium"])
+def test_rebin_filter_matches_oracle(ctx, oracle, filt):
+    w = S.tiny(filter=filt)
+    sc, rc = w.scan, w.recon
+    say("rebin_filter", filt)
+    raw_t = _raw_gpu(ctx, w, 0.0)
+    p = _plan(ctx, w)
+    filt_t, _ = p.alloc_filtered()
+    p.rebin_filter(raw_t, filt_t)
+    p.synchronize()
+    got = to_np(p.filtered_view(filt_t))[..., : sc.n_channels]
+    raw = to_np(raw_t)
+    d = oracle.geometry(sc, rc)
+    ref = oracle.rebin(sc, raw)
+    h = oracle.filter_kernel(d.nfft, d.ds, rc.filter_kind, rc.filter_c, rc.filter_a)
+    ref = oracle.filter_rows(ref, h, d.nfft, use_fft=False)
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(got - ref)) < 2e-6 * scale
+
+
+def test_selection_bit_exact_counts(ctx, oracle):
+    w = S.tiny(nx=48, n_slices=12)
+    sc, rc = w.scan, w.recon
+    d = oracle.geometry(sc, rc)
+    for tz in (8, 16, 32):
+        p = _plan(ctx, w, tile_z=tz)
+        counts = torch.zeros((d.n_slices_native, rc.ny, rc.nx), dtype=torch.int32, device="cuda")
+        total = p.count_voxel_views(counts)
+        ref, ref_total = oracle.bp_counts(sc, rc, 0, d.n_slices_native)
+        say("counts tz", tz, total, ref_total)
+        assert total == ref_total
+        assert np.array_equal(to_np(counts), ref)
+
+
+def test_selection_bit_exact_per_view(ctx, oracle):
+    w = S.small()
+    sc, rc = w.scan, w.recon
+    p = _plan(ctx, w)
+    for (ix, iy, iz) in [(128, 128, 31), (20, 200, 5), (250, 3, 60), (77, 180, 40)]:
+        got = to_np(p.voxel_selection(ix, iy, iz))
+        ref = oracle.bp_selection(sc, rc, ix, iy, iz)
+        say("selection", ix, iy, iz, int((ref >= 0).sum()))
+        assert np.array_equal(got, ref)
+
+
+def test_backproject_matches_oracle(ctx, oracle):
+    # Same filtered input to both (the GPU's FP32 filtered rows): BP only.
+    w = S.tiny(nx=64, n_slices=16, n_channels=192, n_rows=16, ppr=192, rotations=3)
+    sc, rc = w.scan, w.recon
+    d = oracle.geometry(sc, rc)
+    raw_t = _raw_gpu(ctx, w)
+    p = _plan(ctx, w)
+    filt_t, _ = p.alloc_filtered()
+    p.rebin_filter(raw_t, filt_t)
+    nat = p.alloc_native()
+    p.backproject(filt_t, None, nat)
+    p.synchronize()
+    filt = to_np(p.filtered_view(filt_t))[..., : sc.n_channels].astype(np.float64)
+    ref, _ = oracle.backproject(sc, rc, filt, 0, 0, d.n_slices_native)
+    rmse, mx = hu_errors(to_np(nat), ref, fov_mask(rc, d.acq_fov))
+    say("bp tiny rmse %.4f max %.4f HU" % (rmse, mx))
+    assert rmse <= 0.5 and mx <= 2.0
+
+
+def test_thicken_matches_oracle(ctx, oracle):
+    w = S.tiny(nx=32, n_slices=12)
+    d0 = oracle.geometry(w.scan, w.recon)
+    rc = w.recon.replace(slice_thickness=2.5 * d0.csw, slice_pitch=1.5 * d0.csw)
+    w = S.Workload("thk", w.scan, rc, w.phantom)
+    p = _plan(ctx, w)
+    rng = np.random.default_rng(0)
+    native = rng.standard_normal((p.info.n_slices_native, rc.ny, rc.nx)).astype(np.float32)
+    vol = p.alloc_volume()
+    p.thicken(torch.from_numpy(native).cuda(), vol)
+    p.synchronize()
+    ref = oracle.thicken(w.scan, rc, native.astype(np.float64))
+    assert np.max(np.abs(to_np(vol) - ref)) < 1e-5
+
+
+def test_small_workload_end_to_end(ctx, oracle):
+    """736x16, 2304 views, 256^2 x 32 slices at 1.25 mm, Shepp-Logan."""
+    w = S.small()
+    sc, rc = w.scan, w.recon
+    d = oracle.geometry(sc, rc)
+    say("small e2e: native", d.n_slices_native, "requested", d.n_slices_requested)
+    raw_t = _raw_gpu(ctx, w)
+    p = _plan(ctx, w)
+    vol = p.alloc_volume()
+    p.reconstruct(raw_t, vol)
+    p.synchronize()
+    raw = to_np(raw_t)
+    ref_nat, n_sel = oracle.reconstruct_native(sc, rc, raw, 0, d.n_slices_native)
+    ref = oracle.thicken(sc, rc, ref_nat)
+    mask = fov_mask(rc, d.acq_fov)
+    rmse, mx = hu_errors(to_np(vol), ref, mask)
+    say("small e2e rmse %.4f max %.4f HU (voxel-views %d)" % (rmse, mx, n_sel))
+    assert rmse <= 0.5 and mx <= 2.0
+    assert p.count_voxel_views() == n_sel
+
+
+@pytest.mark.parametrize("precision,rmse_bound", [("fp16", 0.5), ("bf16", 8.0)])
+def test_mixed_precision_error(ctx, oracle, precision, rmse_bound):
+    w = S.tiny(nx=64, n_slices=16, n_channels=192, n_rows=16, ppr=192, rotations=3)
+    sc, rc = w.scan, w.recon
+    d = oracle.geometry(sc, rc)
+    raw_t = _raw_gpu(ctx, w, 5e4)
+    ref, _ = oracle.reconstruct_native(sc, rc, to_np(raw_t), 0, d.n_slices_native)
+    p = _plan(ctx, w, precision=precision)
+    filt_t, sc_t = p.alloc_filtered()
+    p.rebin_filter(raw_t, filt_t, sc_t)
+    nat = p.alloc_native()
+    p.backproject(filt_t, sc_t, nat)
+    p.synchronize()
+    rmse, mx = hu_errors(to_np(nat), ref, fov_mask(rc, d.acq_fov))
+    say(f"{precision}: rmse {rmse:.4f} max {mx:.4f} HU")
+    assert rmse <= rmse_bound
+
+
+def test_error_paths(ctx):
+    from freect_amd import wfbp
+    w = S.tiny()
+    with pytest.raises(wfbp.FctError, match="views_per_rotation"):
+        wfbp.Plan(ctx, w.scan.replace(views_per_rotation=95), w.recon)
+    with pytest.raises(wfbp.FctError, match="nx, ny"):
+        wfbp.Plan(ctx, w.scan, w.recon.replace(nx=0))
+    with pytest.raises(wfbp.FctError, match="tile_z"):
+        wfbp.Plan(ctx, w.scan, w.recon, tile_z=12)
+    p = wfbp.Plan(ctx, w.scan, w.recon, precision="fp16")
+    with pytest.raises(wfbp.FctError, match="row_scale"):
+        p.backproject(1, None, 1)
