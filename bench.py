@@ -80,13 +80,13 @@ def cpu_baseline(w, raw_np, n_vv_total, n_native):
     t = time.perf_counter()
     par = pyoracle.rebin(sc, raw_np, 4000, 4096)
     h = pyoracle.filter_kernel(d.nfft, d.ds, rc.filter_kind, rc.filter_c, rc.filter_a)
-    pyoracle.filter_rows(par, h, d.nfft, use_fft=True)
+    par = pyoracle.filter_rows(par, h, d.nfft, use_fft=True)
     t_rf = (time.perf_counter() - t) * sc.n_views / 96.0
     # (b) backprojection of one central native slice.
     s0 = n_native // 2
     v0, v1 = pyoracle.view_range(sc, rc, s0, s0 + 1)
     par = pyoracle.rebin(sc, raw_np, v0, v1)
-    pyoracle.filter_rows(par, h, d.nfft, use_fft=True)
+    par = pyoracle.filter_rows(par, h, d.nfft, use_fft=True)
     t = time.perf_counter()
     vol, n_sel = pyoracle.backproject(sc, rc, par, v0, s0, s0 + 1)
     t_bp1 = time.perf_counter() - t

@@ -214,20 +214,9 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   p->scan = *scan;
   p->recon = *recon;
   if (opts) p->opts = *opts;
-  if (stream) {
-    p->stream = static_cast<hipStream_t>(stream);
-  } else {
-    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-      delete p;
-      set_err(ctx, "hipStreamCreate failed: %s", hipGetErrorString(e));
-      return FCT_ERR_DEVICE;
-    }
-    p->own_stream = true;
-  }
+  p->stream = static_cast<hipStream_t>(stream);   // NULL: the null stream
   auto fail = [&](int code) {
     free_plan_buffers(p);
-    if (p->own_stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return code;
   };
@@ -376,7 +365,6 @@ FCT_API int fct_wfbp_gpu_plan_destroy(fct_wfbp_gpu_plan* p) {
   (void)hipSetDevice(p->device);
   (void)hipStreamSynchronize(p->stream);
   free_plan_buffers(p);
-  if (p->own_stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return FCT_OK;
 }
@@ -403,17 +391,7 @@ FCT_API int fct_wfbp_gpu_plan_info(const fct_wfbp_gpu_plan* p, fct_plan_info* in
 
 FCT_API int fct_wfbp_gpu_set_stream(fct_wfbp_gpu_plan* p, void* stream) {
   if (!p) return FCT_ERR_INVALID_ARGUMENT;
-  if (p->own_stream) {
-    (void)hipStreamSynchronize(p->stream);
-    (void)hipStreamDestroy(p->stream);
-    p->own_stream = false;
-  }
-  if (stream) {
-    p->stream = static_cast<hipStream_t>(stream);
-  } else {
-    HIP_TRY(p->ctx, hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-    p->own_stream = true;
-  }
+  p->stream = static_cast<hipStream_t>(stream);
   return FCT_OK;
 }
 

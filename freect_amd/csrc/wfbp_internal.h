@@ -91,7 +91,6 @@ struct fct_wfbp_gpu_plan {
   fct_wfbp_gpu_ctx* ctx = nullptr;
   int device = 0;
   hipStream_t stream = nullptr;
-  bool own_stream = false;
   fct_scan_geometry scan{};
   fct_recon_config recon{};
   fct_exec_options opts{};

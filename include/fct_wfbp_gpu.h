@@ -122,7 +122,8 @@ FCT_API const char* fct_wfbp_gpu_last_error(const fct_wfbp_gpu_ctx* ctx);
 /* Setup: ConfigureCTGeometry + AllocateKeyArrays + generate_filter
  * (ReconstructionMachine.cu:44-139, 478-544, recon_structs.h:69-83).
  * FP64 view tables and slice grids, FP32 per-chunk offsets, filter spectrum,
- * rebinning tables.  stream may be NULL (the plan creates its own). */
+ * rebinning tables.  stream is the hipStream_t the plan's work is queued on;
+ * NULL is the device's null stream. */
 FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* scan,
                                      const fct_recon_config* recon,
                                      const fct_exec_options* opts, void* stream,

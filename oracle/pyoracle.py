@@ -76,6 +76,7 @@ def lib():
                                              C.c_int, C.c_int, fp, i64p]),
             "or_thicken": (C.c_int, [P(OrScan), P(OrRecon), dp, dp]),
             "or_bp_counts": (C.c_int, [P(OrScan), P(OrRecon), C.c_int, C.c_int, i32p, i64p]),
+            "or_bp_coverage": (C.c_int, [P(OrScan), P(OrRecon), C.c_int, C.c_int, dp]),
             "or_bp_selection": (C.c_int, [P(OrScan), P(OrRecon), C.c_int, C.c_int, C.c_int, i32p]),
             "or_project_phantom": (None, [P(OrScan), C.c_double, C.c_double, P(OrEllipsoid),
                                           C.c_int, fp, C.c_int, C.c_int]),
@@ -189,6 +190,29 @@ def bp_counts(scan, rc, s0, s1):
     s, r = scan_struct(scan), recon_struct(rc)
     lib().or_bp_counts(C.byref(s), C.byref(r), s0, s1, _ptr(counts, C.c_int32), C.byref(n))
     return counts, n.value
+
+
+def bp_coverage(scan, rc, s0, s1) -> np.ndarray:
+    """Per native voxel: min over theta~ of the weight sum of selected views."""
+    out = np.zeros((s1 - s0, rc.ny, rc.nx), np.float64)
+    s, r = scan_struct(scan), recon_struct(rc)
+    lib().or_bp_coverage(C.byref(s), C.byref(r), s0, s1, _ptr(out, C.c_double))
+    return out
+
+
+def covered_mask(scan, rc, threshold=0.5) -> np.ndarray:
+    """Requested-grid voxels whose contributing native slices are all fully
+    covered (every theta~ has weight sum >= threshold)."""
+    d = geometry(scan, rc)
+    cov = bp_coverage(scan, rc, 0, d.n_slices_native) >= threshold
+    zn = slice_grid(rc.start_pos, rc.end_pos, d.csw)
+    pitch = rc.slice_pitch if rc.slice_pitch > 0 else rc.slice_thickness
+    zr = slice_grid(rc.start_pos, rc.end_pos, pitch)
+    out = np.ones((len(zr), rc.ny, rc.nx), bool)
+    for s, z in enumerate(zr):
+        used = np.abs(zn - z) < rc.slice_thickness
+        out[s] = np.all(cov[used], axis=0) if used.any() else False
+    return out
 
 
 def bp_selection(scan, rc, ix, iy, iz) -> np.ndarray:

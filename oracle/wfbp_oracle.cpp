@@ -120,7 +120,8 @@ SliceGrid make_slices(const Geo& G, const or_recon* r) {
 // T = double: weights from FP64 geometry.  T = float: the FP32 CPU baseline.
 template <typename T, typename P>
 int bp_core(const or_scan* s, const or_recon* r, const P* filtered, int v0,
-            int v1, int s0, int s1, T* vol, int64_t* n_sel, int32_t* counts) {
+            int v1, int s0, int s1, T* vol, int64_t* n_sel, int32_t* counts,
+            double* minden = nullptr) {
   const Geo G = make_geo(s);
   const SliceGrid sg = make_slices(G, r);
   const int nn = (int)sg.zn.size();
@@ -170,8 +171,10 @@ int bp_core(const or_scan* s, const or_recon* r, const P* filtered, int v0,
   for (int col = 0; col < nx * ny; ++col) {
     const int ix = col % nx, iy = col / nx;
     std::vector<T> acc(ns, (T)0), num(ns), den(ns);
+    std::vector<double> mden(ns, 0.0);
     std::vector<int32_t> cnt(ns, 0);
     if (fct_in_fov(xs[ix], ys[iy], g)) {
+      std::fill(mden.begin(), mden.end(), 1e300);
       for (int t = 0; t < H; ++t) {
         std::fill(num.begin(), num.end(), (T)0);
         std::fill(den.begin(), den.end(), (T)0);
@@ -222,13 +225,16 @@ int bp_core(const or_scan* s, const or_recon* r, const P* filtered, int v0,
             cnt[i] += 1;
           }
         }
-        for (int i = 0; i < ns; ++i)
+        for (int i = 0; i < ns; ++i) {
           if (den[i] > (T)0) acc[i] += num[i] / den[i];   // den = 0: no contribution
+          mden[i] = std::min(mden[i], (double)den[i]);
+        }
       }
     }
     for (int i = 0; i < ns; ++i) {
       if (vol) vol[(size_t)i * nx * ny + col] = (T)(scale * (double)acc[i]);
       if (counts) counts[(size_t)i * nx * ny + col] = cnt[i];
+      if (minden) minden[(size_t)i * nx * ny + col] = mden[i];
       total += cnt[i];
     }
   }
@@ -377,6 +383,13 @@ int or_bp_counts(const or_scan* s, const or_recon* r, int s0, int s1, int32_t* c
   const Geo G = make_geo(s);
   return bp_core<double, double>(s, r, nullptr, 0, G.nviews, s0, s1, nullptr,
                                  n_selected, counts);
+}
+
+int or_bp_coverage(const or_scan* s, const or_recon* r, int s0, int s1, double* minden) {
+  // Weights only (no filtered data): min over theta~ of sum_k W.
+  const Geo G = make_geo(s);
+  return bp_core<double, double>(s, r, nullptr, 0, G.nviews, s0, s1, nullptr, nullptr, nullptr,
+                                 minden);
 }
 
 int or_thicken(const or_scan* s, const or_recon* r, const double* native, double* out) {

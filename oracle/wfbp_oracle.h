@@ -82,6 +82,10 @@ int or_thicken(const or_scan* s, const or_recon* r, const double* native,
 // Selected-sample count per native voxel of slices [s0, s1) (layout like vol).
 int or_bp_counts(const or_scan* s, const or_recon* r, int s0, int s1,
                  int32_t* counts, int64_t* n_selected);
+// Coverage per native voxel of slices [s0, s1): min over theta~ of the FP64
+// weight sum sum_k W(q_k) of the selected views (0 where some theta~ is empty).
+int or_bp_coverage(const or_scan* s, const or_recon* r, int s0, int s1,
+                   double* minden);
 // One (voxel column, slice) of the selection rule, for bit-exact comparison:
 // writes, for every parallel view, qf*65536+pf or -1 when unused.
 int or_bp_selection(const or_scan* s, const or_recon* r, int ix, int iy,
