@@ -20,7 +20,8 @@ LIB_NAME = "libfct_wfbp_gpu.so"
 LIB_PATH = os.path.join(LIBDIR, LIB_NAME)
 ARCH = os.environ.get("FCT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["capi.hip", "rebin_filter.hip", "backproject.hip", "aux_kernels.hip"]
+SOURCES = ["capi.hip", "rebin_filter.hip", "backproject.hip", "backproject_v1.hip",
+           "aux_kernels.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
@@ -43,16 +44,45 @@ def _compile(src: str) -> str:
     return obj
 
 
+# Host C++ above the C ABI (mirror of FreeCT's ReconstructionMachine,
+# parse_config and RawDataSet), compiled by the host compiler.
+HOST_SOURCES = ["host/fct_host.cpp"]
+CLI_SOURCE = "host/main.cpp"
+CLI_PATH = os.path.join(LIBDIR, "fct_wfbp")
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall",
+             "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+
+
+def _compile_host(src: str) -> str:
+    obj = os.path.join(BUILD, os.path.basename(os.path.splitext(src)[0]) + ".host.o")
+    cmd = [shutil.which("g++") or "g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
 def build_library(jobs: int = 4) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
+        host_objs = list(ex.map(_compile_host, HOST_SOURCES))
     tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, *host_objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB_PATH)
+    main_obj = _compile_host(CLI_SOURCE)
+    cmd = [shutil.which("g++") or "g++", "-o", CLI_PATH + ".tmp", main_obj, *host_objs,
+           "-L" + LIBDIR,
+           "-lfct_wfbp_gpu", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(CLI_PATH + ".tmp", CLI_PATH)
     return LIB_PATH
 
 

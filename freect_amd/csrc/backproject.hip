@@ -224,7 +224,8 @@ hipError_t launch_tz(const BpArgs& a, int precision, int mode, hipStream_t s) {
 
 hipError_t launch_backproject(const BpArgs& a, int tile_z, int variant, int precision, int mode,
                               hipStream_t s) {
-  (void)variant;
+  if (variant != FCT_BP_DIRECT && mode != kBpSelect)
+    return launch_backproject_v1(a, tile_z, precision, mode, s);
   switch (tile_z) {
     case 8: return launch_tz<8>(a, precision, mode, s);
     case 16: return launch_tz<16>(a, precision, mode, s);

@@ -126,6 +126,8 @@ int validate(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* s, const fct_recon_
       return set_err(ctx, "unknown precision %d", o->precision), FCT_ERR_INVALID_ARGUMENT;
     if (o->tile_z != 0 && o->tile_z != 8 && o->tile_z != 16 && o->tile_z != 32)
       return set_err(ctx, "tile_z must be 0, 8, 16 or 32"), FCT_ERR_INVALID_ARGUMENT;
+    if (o->bp_variant < FCT_BP_AUTO || o->bp_variant > FCT_BP_LEAN)
+      return set_err(ctx, "unknown bp_variant %d", o->bp_variant), FCT_ERR_INVALID_ARGUMENT;
   }
   return FCT_OK;
 }
@@ -254,7 +256,9 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
                                s.central_channel, s.central_row, s.feed);
   p->scale = (float)(2.0 * kPi / (double)p->ppr);
   p->tile_z = (opts && opts->tile_z) ? opts->tile_z : 16;
-  p->variant = opts ? opts->bp_variant : 0;
+  p->variant = (opts && opts->bp_variant) ? opts->bp_variant : FCT_BP_LEAN;
+  if ((size_t)p->nviews * p->nrows * p->cpitch >= ((size_t)1 << 31))
+    return fail((set_err(ctx, "filtered projections exceed 2^31 elements"), FCT_ERR_LIMIT));
   if (p->nfft > 4096) return fail((set_err(ctx, "nfft > 4096 unsupported"), FCT_ERR_LIMIT));
 
   // Slice grids (ReconstructionMachine.cu:107-137) in FP64.

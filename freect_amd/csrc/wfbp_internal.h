@@ -75,6 +75,8 @@ struct PhantomArgs {
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s);
 hipError_t launch_backproject(const BpArgs& a, int tile_z, int variant, int precision, int mode,
                               hipStream_t s);
+hipError_t launch_backproject_v1(const BpArgs& a, int tile_z, int precision, int mode,
+                                 hipStream_t s);
 hipError_t launch_thicken(const ThickenArgs& a, hipStream_t s);
 hipError_t launch_phantom(const PhantomArgs& a, hipStream_t s);
 

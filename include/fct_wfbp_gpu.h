@@ -77,10 +77,17 @@ typedef struct {
   double adaptive_filtration_s;
 } fct_recon_config;
 
+/* Backprojection kernel variants (identical selection and arithmetic). */
+typedef enum {
+  FCT_BP_AUTO = 0,             /* tuned default */
+  FCT_BP_DIRECT = 1,           /* one wave per 8x8xTZ tile, taps read per sample */
+  FCT_BP_LEAN = 2              /* + wave-uniform slice skipping, buffer loads */
+} fct_bp_variant;
+
 typedef struct {
   int32_t precision;           /* fct_precision */
   int32_t tile_z;              /* slices per backprojection tile: 0 = tuned default, or 8/16/32 */
-  int32_t bp_variant;          /* 0 = tuned default; see fct_wfbp_gpu_plan_info */
+  int32_t bp_variant;          /* fct_bp_variant */
   int32_t reserved;
 } fct_exec_options;
 
