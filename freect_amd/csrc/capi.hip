@@ -136,7 +136,8 @@ void free_plan_buffers(fct_wfbp_gpu_plan* p) {
   void* bufs[] = {p->d_xs,       p->d_ys,        p->d_cs,        p->d_zs,       p->d_zt,
                   p->d_slab_vrange, p->d_rebin,  p->d_Hn,        p->d_tw,       p->d_thk_off,
                   p->d_thk_idx,  p->d_thk_w,     p->d_filtered,  p->d_row_scale, p->d_native,
-                  p->d_raw_stage, p->d_vol_stage};
+                  p->d_raw_stage, p->d_vol_stage, p->d_gbar, p->d_thkT_off, p->d_thkT_idx,
+                  p->d_thkT_w};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
 }
@@ -342,9 +343,28 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
     }
     toff[so + 1] = (int32_t)tidx.size();
   }
+  // Transposed table: per native slice, the requested slices and weights.
+  std::vector<int32_t> tToff(p->n_native + 1, 0), tTidx;
+  std::vector<float> tTw;
+  {
+    std::vector<std::vector<std::pair<int, float>>> per(p->n_native);
+    for (int so = 0; so < p->n_req; ++so)
+      for (int k = toff[so]; k < toff[so + 1]; ++k) per[tidx[k]].emplace_back(so, tw8[k]);
+    for (int i = 0; i < p->n_native; ++i) {
+      for (auto& pr : per[i]) {
+        tTidx.push_back(pr.first);
+        tTw.push_back(pr.second);
+      }
+      tToff[i + 1] = (int32_t)tTidx.size();
+    }
+  }
   if (tidx.empty()) {
     tidx.push_back(0);
     tw8.push_back(0.0f);
+  }
+  if (tTidx.empty()) {
+    tTidx.push_back(0);
+    tTw.push_back(0.0f);
   }
 
   PLAN_TRY(upload(&p->d_xs, xs, p->stream));
@@ -359,6 +379,9 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   PLAN_TRY(upload(&p->d_thk_off, toff, p->stream));
   PLAN_TRY(upload(&p->d_thk_idx, tidx, p->stream));
   PLAN_TRY(upload(&p->d_thk_w, tw8, p->stream));
+  PLAN_TRY(upload(&p->d_thkT_off, tToff, p->stream));
+  PLAN_TRY(upload(&p->d_thkT_idx, tTidx, p->stream));
+  PLAN_TRY(upload(&p->d_thkT_w, tTw, p->stream));
 #undef PLAN_TRY
   *out = p;
   return FCT_OK;
@@ -452,6 +475,87 @@ FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* p, const float* native, floa
   a.n_req = p->n_req;
   HIP_TRY(p->ctx, fct::launch_thicken(a, p->stream));
   return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_filter(fct_wfbp_gpu_plan* p, const float* lines_in, float* lines_out) {
+  if (!p || !lines_in || !lines_out) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  fct::RebinFilterArgs a{};
+  a.lines_in = lines_in;
+  a.out = lines_out;
+  a.Hn = p->d_Hn;
+  a.tw = p->d_tw;
+  a.tab = p->d_rebin;
+  a.nviews = p->nviews;
+  a.nrows = p->nrows;
+  a.ncols = p->ncols;
+  a.cpitch = p->cpitch;
+  a.nfft = p->nfft;
+  a.log2n = p->log2n;
+  a.precision = FCT_PRECISION_FP32;
+  a.nlines = p->nviews * p->nrows;
+  HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_backproject_adjoint(fct_wfbp_gpu_plan* p, const float* native_bar,
+                                             float* filtered_bar) {
+  if (!p || !native_bar || !filtered_bar) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  HIP_TRY(p->ctx, fct::launch_fill(filtered_bar, (size_t)p->nviews * p->nrows * p->cpitch, 0.0f,
+                                   p->stream));
+  fct::BpAdjArgs a{};
+  a.bp = bp_args(p);
+  a.xbar = native_bar;
+  a.gbar = filtered_bar;
+  HIP_TRY(p->ctx, fct::launch_backproject_adjoint(a, p->tile_z, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_rebin_adjoint(fct_wfbp_gpu_plan* p, const float* filtered_bar,
+                                       float* raw_bar) {
+  if (!p || !filtered_bar || !raw_bar) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  HIP_TRY(p->ctx, fct::launch_fill(raw_bar, (size_t)p->nviews * p->nrows * p->ncols, 0.0f,
+                                   p->stream));
+  fct::RebinAdjArgs a{};
+  a.gbar = filtered_bar;
+  a.pbar = raw_bar;
+  a.tab = p->d_rebin;
+  a.nviews = p->nviews;
+  a.nrows = p->nrows;
+  a.ncols = p->ncols;
+  a.cpitch = p->cpitch;
+  HIP_TRY(p->ctx, fct::launch_rebin_adjoint(a, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_thicken_adjoint(fct_wfbp_gpu_plan* p, const float* volume_bar,
+                                         float* native_bar) {
+  if (!p || !volume_bar || !native_bar) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  fct::ThickenArgs a{};
+  a.native = volume_bar;
+  a.out = native_bar;
+  a.off = p->d_thkT_off;
+  a.idx = p->d_thkT_idx;
+  a.w = p->d_thkT_w;
+  a.plane = p->recon.nx * p->recon.ny;
+  a.n_req = p->n_native;
+  HIP_TRY(p->ctx, fct::launch_thicken(a, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_adjoint(fct_wfbp_gpu_plan* p, const float* native_bar, float* raw_bar) {
+  if (!p || !native_bar || !raw_bar) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  if (!p->d_gbar)
+    HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_gbar),
+                              (size_t)p->nviews * p->nrows * p->cpitch * sizeof(float)));
+  int st = fct_wfbp_gpu_backproject_adjoint(p, native_bar, p->d_gbar);
+  if (!st) st = fct_wfbp_gpu_filter(p, p->d_gbar, p->d_gbar);
+  if (!st) st = fct_wfbp_gpu_rebin_adjoint(p, p->d_gbar, raw_bar);
+  return st;
 }
 
 FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* p, const float* raw, float* volume,

@@ -85,7 +85,12 @@ __global__ __launch_bounds__(kRfThreads) void rebin_filter_kernel(RebinFilterArg
 
   for (int c = threadIdx.x; c < N; c += kRfThreads) {
     float va = 0.0f, vb = 0.0f;
-    if (c < a.ncols) {
+    if (a.lines_in) {   // filter-only: the lines are already parallel rows
+      if (c < a.ncols) {
+        va = a.lines_in[(size_t)l0 * a.cpitch + c];
+        if (has1) vb = a.lines_in[(size_t)l1 * a.cpitch + c];
+      }
+    } else if (c < a.ncols) {
       const RebinDev e = a.tab[c];
       if (e.valid) {   // fan channel outside [0, ncols): 0 (cuda_kernels.cuh:32-34)
         va = rebin_sample(a.raw, j0, r0, e, a.nviews, a.nrows, a.ncols);

@@ -29,6 +29,7 @@ struct RebinFilterArgs {
   int32_t nviews, nrows, ncols, cpitch, nfft, log2n;
   int32_t precision;
   int32_t nlines;            // nviews * nrows
+  const float* lines_in;     // filter-only mode: [line][cpitch] FP32 input (no rebinning)
 };
 
 // Everything the backprojection kernel reads.
@@ -72,7 +73,23 @@ struct PhantomArgs {
   unsigned long long seed;
 };
 
+struct BpAdjArgs {
+  BpArgs bp;                 // geometry tables (filt/native unused)
+  const float* xbar;         // [slice][y][x] native-volume cotangent
+  float* gbar;               // [view][row][cpitch] FP32, accumulated into
+};
+
+struct RebinAdjArgs {
+  const float* gbar;         // [view][row][cpitch]
+  float* pbar;               // [view][row][ncols], accumulated into
+  const RebinDev* tab;
+  int32_t nviews, nrows, ncols, cpitch;
+};
+
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s);
+hipError_t launch_backproject_adjoint(const BpAdjArgs& a, int tile_z, hipStream_t s);
+hipError_t launch_rebin_adjoint(const RebinAdjArgs& a, hipStream_t s);
+hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_backproject(const BpArgs& a, int tile_z, int variant, int precision, int mode,
                               hipStream_t s);
 hipError_t launch_backproject_v1(const BpArgs& a, int tile_z, int precision, int mode,
@@ -123,6 +140,10 @@ struct fct_wfbp_gpu_plan {
   float* d_native = nullptr;
   float* d_raw_stage = nullptr;
   float* d_vol_stage = nullptr;
+  float* d_gbar = nullptr;          // adjoint scratch, [view][row][cpitch] FP32
+  int32_t* d_thkT_off = nullptr;    // transposed thickening table
+  int32_t* d_thkT_idx = nullptr;
+  float* d_thkT_w = nullptr;
 
   size_t filtered_bytes() const {
     const size_t el = opts.precision == FCT_PRECISION_FP32 ? 4 : 2;

@@ -78,6 +78,11 @@ SIGNATURES = {
     "fct_wfbp_gpu_backproject": (C.c_int, [_VP, _VP, _VP, _VP]),
     "fct_wfbp_gpu_thicken": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_reconstruct": (C.c_int, [_VP, _VP, _VP, C.c_int32]),
+    "fct_wfbp_gpu_backproject_adjoint": (C.c_int, [_VP, _VP, _VP]),
+    "fct_wfbp_gpu_filter": (C.c_int, [_VP, _VP, _VP]),
+    "fct_wfbp_gpu_rebin_adjoint": (C.c_int, [_VP, _VP, _VP]),
+    "fct_wfbp_gpu_thicken_adjoint": (C.c_int, [_VP, _VP, _VP]),
+    "fct_wfbp_gpu_adjoint": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_count_voxel_views": (C.c_int, [_VP, _VP, _P(C.c_int64)]),
     "fct_wfbp_gpu_voxel_selection": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _VP]),
     "fct_wfbp_gpu_synchronize": (C.c_int, [_VP]),
@@ -271,6 +276,47 @@ class Plan:
         self.ctx.check(load_library().fct_wfbp_gpu_reconstruct(
             self.h, raw.ctypes.data, vol.ctypes.data, 1), "reconstruct(host)")
         return vol
+
+    # -- adjoint (transpose of rebin -> filter -> backproject, FP32) ----------
+    def alloc_raw(self):
+        import torch
+        return torch.empty((self.scan.n_views, self.scan.n_rows, self.scan.n_channels),
+                           dtype=torch.float32, device=self._dev())
+
+    def alloc_lines(self):
+        """FP32 [view][row][channel_pitch] buffer (filtered rows and their cotangents)."""
+        import torch
+        return torch.empty((self.scan.n_views, self.scan.n_rows, self.info.channel_pitch),
+                           dtype=torch.float32, device=self._dev())
+
+    def backproject_adjoint(self, native_bar, lines_bar):
+        self.ctx.check(load_library().fct_wfbp_gpu_backproject_adjoint(
+            self.h, _ptr(native_bar), _ptr(lines_bar)), "backproject_adjoint")
+
+    def filter_lines(self, lines_in, lines_out):
+        self.ctx.check(load_library().fct_wfbp_gpu_filter(self.h, _ptr(lines_in), _ptr(lines_out)),
+                       "filter")
+
+    def rebin_adjoint(self, lines_bar, raw_bar):
+        self.ctx.check(load_library().fct_wfbp_gpu_rebin_adjoint(self.h, _ptr(lines_bar),
+                                                                _ptr(raw_bar)), "rebin_adjoint")
+
+    def thicken_adjoint(self, volume_bar, native_bar):
+        self.ctx.check(load_library().fct_wfbp_gpu_thicken_adjoint(self.h, _ptr(volume_bar),
+                                                                  _ptr(native_bar)),
+                       "thicken_adjoint")
+
+    def adjoint(self, native_bar, raw_bar):
+        self.ctx.check(load_library().fct_wfbp_gpu_adjoint(self.h, _ptr(native_bar),
+                                                          _ptr(raw_bar)), "adjoint")
+
+    def forward_native(self, raw, native, filt=None):
+        """raw -> native volume (rebin + filter + backproject), FP32 plan."""
+        if filt is None:
+            filt, _ = self.alloc_filtered()
+        self.rebin_filter(raw, filt)
+        self.backproject(filt, None, native)
+        return filt
 
     def count_voxel_views(self, per_voxel=None) -> int:
         tot = C.c_int64(0)

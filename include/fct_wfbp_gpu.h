@@ -162,6 +162,24 @@ FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* plan, const float* native, f
 FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* plan, const float* raw, float* volume,
                                      int32_t host_buffers);
 
+/* Adjoint (no reference counterpart): the exact transpose of the FP32 chain
+ * rebin -> filter -> backproject, i.e. a matched helical forward projector.
+ * Buffers are device pointers in the forward layouts; outputs are
+ * overwritten.  Stage transposes:
+ *   backproject_adjoint: native [slice][y][x] -> FP32 filtered [view][row][channel_pitch]
+ *   filter:              FP32 filtered rows -> filtered rows (self-adjoint; in place allowed)
+ *   rebin_adjoint:       FP32 filtered rows -> raw [view][row][ch]
+ *   thicken_adjoint:     volume [slice][y][x] -> native [slice][y][x]
+ * adjoint runs the first three with plan-owned scratch. */
+FCT_API int fct_wfbp_gpu_backproject_adjoint(fct_wfbp_gpu_plan* plan, const float* native_bar,
+                                             float* filtered_bar);
+FCT_API int fct_wfbp_gpu_filter(fct_wfbp_gpu_plan* plan, const float* lines_in, float* lines_out);
+FCT_API int fct_wfbp_gpu_rebin_adjoint(fct_wfbp_gpu_plan* plan, const float* filtered_bar,
+                                       float* raw_bar);
+FCT_API int fct_wfbp_gpu_thicken_adjoint(fct_wfbp_gpu_plan* plan, const float* volume_bar,
+                                         float* native_bar);
+FCT_API int fct_wfbp_gpu_adjoint(fct_wfbp_gpu_plan* plan, const float* native_bar, float* raw_bar);
+
 /* Exact count of selected (voxel, view) samples, the throughput unit.
  * per_voxel (optional, device, native_elems int32) receives the count of
  * every native voxel; *total the sum. Synchronous. */

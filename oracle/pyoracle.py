@@ -84,6 +84,10 @@ def lib():
                                                 dp, i64p]),
             "or_view_range": (None, [P(OrScan), P(OrRecon), C.c_int, C.c_int, P(C.c_int),
                                      P(C.c_int)]),
+            "or_backproject_adjoint": (C.c_int, [P(OrScan), P(OrRecon), dp, C.c_int, C.c_int,
+                                                 C.c_int, C.c_int, dp]),
+            "or_rebin_adjoint": (None, [P(OrScan), dp, dp, C.c_int, C.c_int]),
+            "or_adjoint_native": (C.c_int, [P(OrScan), P(OrRecon), dp, C.c_int, C.c_int, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -259,3 +263,36 @@ def reconstruct_native(scan, rc, raw: np.ndarray, s0: int, s1: int):
     if rc_ != 0:
         raise RuntimeError(f"or_reconstruct_native failed ({rc_})")
     return vol, n.value
+
+
+def backproject_adjoint(scan, rc, xbar: np.ndarray, v0: int, v1: int, s0: int) -> np.ndarray:
+    xbar = np.ascontiguousarray(xbar, np.float64)
+    s1 = s0 + xbar.shape[0]
+    g = np.zeros((v1 - v0, scan.n_rows, scan.n_channels), np.float64)
+    s, r = scan_struct(scan), recon_struct(rc)
+    st = lib().or_backproject_adjoint(C.byref(s), C.byref(r), _ptr(xbar, C.c_double), v0, v1,
+                                      s0, s1, _ptr(g, C.c_double))
+    if st:
+        raise RuntimeError(f"or_backproject_adjoint failed ({st})")
+    return g
+
+
+def rebin_adjoint(scan, gbar: np.ndarray, v0: int = 0) -> np.ndarray:
+    gbar = np.ascontiguousarray(gbar, np.float64)
+    v1 = v0 + gbar.shape[0]
+    p = np.zeros((scan.n_views, scan.n_rows, scan.n_channels), np.float64)
+    s = scan_struct(scan)
+    lib().or_rebin_adjoint(C.byref(s), _ptr(gbar, C.c_double), _ptr(p, C.c_double), v0, v1)
+    return p
+
+
+def adjoint_native(scan, rc, xbar: np.ndarray, s0: int = 0) -> np.ndarray:
+    xbar = np.ascontiguousarray(xbar, np.float64)
+    s1 = s0 + xbar.shape[0]
+    p = np.zeros((scan.n_views, scan.n_rows, scan.n_channels), np.float64)
+    s, r = scan_struct(scan), recon_struct(rc)
+    st = lib().or_adjoint_native(C.byref(s), C.byref(r), _ptr(xbar, C.c_double), s0, s1,
+                                 _ptr(p, C.c_double))
+    if st:
+        raise RuntimeError(f"or_adjoint_native failed ({st})")
+    return p

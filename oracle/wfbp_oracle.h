@@ -103,3 +103,17 @@ void or_view_range(const or_scan* s, const or_recon* r, int s0, int s1,
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+// Transpose of the chain (wfbp_oracle_adjoint.cpp).  gbar / pbar are
+// accumulated into.
+int or_backproject_adjoint(const or_scan* s, const or_recon* r, const double* xbar, int v0,
+                           int v1, int s0, int s1, double* gbar);
+void or_rebin_adjoint(const or_scan* s, const double* gbar, double* pbar, int v0, int v1);
+int or_adjoint_native(const or_scan* s, const or_recon* r, const double* xbar, int s0, int s1,
+                      double* pbar);
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,190 @@
+"""The hand-written adjoint (matched helical forward projector) on the GPU.
+
+Checks, all through the C ABI:
+  * dot-product identity <A p, x> = <p, A^T x> per stage and for the whole
+    chain over 16 seeded Gaussian pairs (FP64 inner products), <= 1e-5 rel;
+  * the GPU adjoint against the FP64 oracle adjoint;
+  * FP64 central finite differences of f(p) = <A_64 p, x> (oracle forward)
+    along 64 random directions against <d, A^T x> from the GPU;
+  * the torch.autograd wrappers' gradients equal the adjoint.
+"""
+import numpy as np
+import pytest
+
+from freect_amd import scans as S
+from helpers import gpu_ctx, say, to_np
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return gpu_ctx()
+
+
+def _tiny():
+    return S.tiny(nx=32, n_slices=8, n_channels=96, n_rows=8, ppr=96, rotations=2.5)
+
+
+def _plan(ctx, w, **kw):
+    from freect_amd import wfbp
+    return wfbp.Plan(ctx, w.scan, w.recon, **kw)
+
+
+def _dot(a, b):
+    return float(torch.dot(a.flatten().double(), b.flatten().double()))
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def test_stage_dot_products(ctx):
+    w = _tiny()
+    p = _plan(ctx, w)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    sc = w.scan
+    nch, cp = sc.n_channels, p.info.channel_pitch
+    for trial in range(4):
+        raw = torch.randn((sc.n_views, sc.n_rows, nch), device="cuda", generator=gen)
+        lines = torch.zeros((sc.n_views, sc.n_rows, cp), device="cuda")
+        lines[..., :nch] = torch.randn((sc.n_views, sc.n_rows, nch), device="cuda", generator=gen)
+        nat = torch.randn((p.info.n_slices_native, w.recon.ny, w.recon.nx), device="cuda",
+                          generator=gen)
+        # filter (self-adjoint)
+        f1, f2 = p.alloc_lines(), p.alloc_lines()
+        lines2 = torch.zeros_like(lines)
+        lines2[..., :nch] = torch.randn((sc.n_views, sc.n_rows, nch), device="cuda", generator=gen)
+        p.filter_lines(lines, f1)
+        p.filter_lines(lines2, f2)
+        torch.cuda.synchronize()
+        a, b = _dot(f1[..., :nch], lines2[..., :nch]), _dot(lines[..., :nch], f2[..., :nch])
+        assert _rel(a, b) < 1e-5, (a, b)
+        # backprojection
+        bp = p.alloc_native()
+        p.backproject(lines, None, bp)
+        gt = p.alloc_lines()
+        p.backproject_adjoint(nat, gt)
+        torch.cuda.synchronize()
+        a, b = _dot(bp, nat), _dot(lines[..., :nch], gt[..., :nch])
+        say("bp dot", a, b, _rel(a, b))
+        assert _rel(a, b) < 1e-5
+        # rebin: forward via rebin_filter on the identity filter is not
+        # exposed; use the whole-chain identity below and the oracle checks.
+        rb = p.alloc_raw()
+        p.rebin_adjoint(lines, rb)
+        torch.cuda.synchronize()
+        assert torch.isfinite(rb).all()
+
+
+def test_whole_chain_dot_products_16_pairs(ctx):
+    w = _tiny()
+    p = _plan(ctx, w)
+    gen = torch.Generator(device="cuda").manual_seed(2024)
+    worst = 0.0
+    for trial in range(16):
+        raw = torch.randn((w.scan.n_views, w.scan.n_rows, w.scan.n_channels), device="cuda",
+                          generator=gen)
+        x = torch.randn((p.info.n_slices_native, w.recon.ny, w.recon.nx), device="cuda",
+                        generator=gen)
+        ax = p.alloc_native()
+        p.forward_native(raw, ax)
+        atx = p.alloc_raw()
+        p.adjoint(x, atx)
+        torch.cuda.synchronize()
+        a, b = _dot(ax, x), _dot(raw, atx)
+        worst = max(worst, _rel(a, b))
+    say("whole-chain dot test: worst rel %.3g over 16 pairs" % worst)
+    assert worst < 1e-5
+
+
+def test_adjoint_workload_dot_products(ctx):
+    """888x64 detector, 3 rotations (2952 views), 512^2 x 64 native slices."""
+    w = S.adjoint()
+    p = _plan(ctx, w)
+    gen = torch.Generator(device="cuda").manual_seed(77)
+    worst = 0.0
+    for trial in range(16):
+        raw = torch.randn((w.scan.n_views, w.scan.n_rows, w.scan.n_channels), device="cuda",
+                          generator=gen)
+        x = torch.randn((p.info.n_slices_native, w.recon.ny, w.recon.nx), device="cuda",
+                        generator=gen)
+        ax = p.alloc_native()
+        p.forward_native(raw, ax)
+        atx = p.alloc_raw()
+        p.adjoint(x, atx)
+        torch.cuda.synchronize()
+        worst = max(worst, _rel(_dot(ax, x), _dot(raw, atx)))
+    say("adjoint workload dot test: worst rel %.3g over 16 pairs" % worst)
+    assert worst < 1e-5
+
+
+def test_adjoint_matches_oracle(ctx, oracle):
+    w = _tiny()
+    sc, rc = w.scan, w.recon
+    p = _plan(ctx, w)
+    d = oracle.geometry(sc, rc)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((d.n_slices_native, rc.ny, rc.nx)).astype(np.float32)
+    atx = p.alloc_raw()
+    p.adjoint(torch.from_numpy(x).cuda(), atx)
+    torch.cuda.synchronize()
+    ref = oracle.adjoint_native(sc, rc, x.astype(np.float64), 0)
+    got = to_np(atx).astype(np.float64)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    say("adjoint vs FP64 oracle adjoint: max rel %.3g" % err)
+    assert err < 1e-4
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 2e-5
+
+
+def test_adjoint_fp64_finite_differences(ctx, oracle):
+    # f(p) = <A_64 p, x>; its directional derivative along d is <d, A^T x>.
+    w = S.tiny(nx=16, n_slices=4, n_channels=48, n_rows=4, ppr=48, rotations=2.5)
+    sc, rc = w.scan, w.recon
+    p = _plan(ctx, w)
+    d = oracle.geometry(sc, rc)
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal((d.n_slices_native, rc.ny, rc.nx))
+    atx = p.alloc_raw()
+    p.adjoint(torch.from_numpy(x.astype(np.float32)).cuda(), atx)
+    torch.cuda.synchronize()
+    g = to_np(atx).astype(np.float64)
+    x32 = x.astype(np.float32).astype(np.float64)
+    p0 = rng.standard_normal((sc.n_views, sc.n_rows, sc.n_channels)).astype(np.float32)
+    eps = 0.5   # A is linear: central differences are exact up to FP64 rounding
+    worst = 0.0
+    for k in range(64):
+        dirn = rng.standard_normal(p0.shape).astype(np.float32)
+        fp, _ = oracle.reconstruct_native(sc, rc, p0 + eps * dirn, 0, d.n_slices_native)
+        fm, _ = oracle.reconstruct_native(sc, rc, p0 - eps * dirn, 0, d.n_slices_native)
+        fd = (np.vdot(fp, x32) - np.vdot(fm, x32)) / (2 * eps)
+        an = np.vdot(dirn.astype(np.float64), g)
+        worst = max(worst, abs(fd - an) / max(abs(fd), 1e-12))
+    say("FP64 central differences vs GPU adjoint: worst rel %.3g over 64 directions" % worst)
+    assert worst < 1e-4
+
+
+def test_autograd_wrappers(ctx):
+    from freect_amd import autograd as AG
+    w = _tiny()
+    p = _plan(ctx, w)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    raw = torch.randn((w.scan.n_views, w.scan.n_rows, w.scan.n_channels), device="cuda",
+                      generator=gen, requires_grad=True)
+    wgt = torch.randn((p.info.n_slices_native, w.recon.ny, w.recon.nx), device="cuda",
+                      generator=gen)
+    (AG.wfbp(raw, p) * wgt).sum().backward()
+    atx = p.alloc_raw()
+    p.adjoint(wgt, atx)
+    torch.cuda.synchronize()
+    assert torch.allclose(raw.grad, atx, rtol=1e-5, atol=1e-6 * atx.abs().max().item())
+    x = wgt.clone().requires_grad_(True)
+    r = torch.randn_like(raw)
+    (AG.project(x, p) * r).sum().backward()
+    ar = p.alloc_native()
+    p.forward_native(r.detach(), ar)
+    torch.cuda.synchronize()
+    assert torch.allclose(x.grad, ar, rtol=1e-5, atol=1e-6 * ar.abs().max().item())
