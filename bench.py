@@ -107,6 +107,9 @@ def main():
     ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tile-z", type=int, default=0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of known "
+                         "traffic, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
@@ -120,6 +123,13 @@ def main():
     from freect_amd import wfbp
 
     torch.cuda.set_device(0)
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib * 2 ** 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
+        del src, dst
     w = S.full()
     ctx = wfbp.Context(0)
     log("generating synthetic scan", w.scan.n_views, w.scan.n_rows, w.scan.n_channels)

@@ -49,7 +49,7 @@ def _compile(src: str) -> str:
 HOST_SOURCES = ["host/fct_host.cpp"]
 CLI_SOURCE = "host/main.cpp"
 CLI_PATH = os.path.join(LIBDIR, "fct_wfbp")
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall",
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-ffp-contract=off",
              "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 

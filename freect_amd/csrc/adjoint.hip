@@ -19,12 +19,6 @@
 namespace fct {
 namespace {
 
-__device__ __forceinline__ float w_of_q(float q) {
-  const float u = fmaxf(fabsf(q) - FCT_WQ, 0.0f) * (1.5707963f / (1.0f - FCT_WQ));
-  const float c = __cosf(u);
-  return c * c;
-}
-
 template <int TZ, bool CENTERED>
 __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
   const BpArgs& b = a.bp;
@@ -101,7 +95,7 @@ __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
             sel = chan_ok && !(q > 1.0f || q < -1.0f) && qfl >= 0.0f &&
                   qfl <= (float)(g.nrows - 2);
           if (!sel) continue;
-          const float W = w_of_q(q);
+          const float W = fct_w_of_q(q);
           if (pass == 0) {
             den[i] += W;
             continue;
@@ -161,6 +155,7 @@ hipError_t launch_tz(const BpAdjArgs& a, hipStream_t s) {
 
 hipError_t launch_backproject_adjoint(const BpAdjArgs& a, int tile_z, hipStream_t s) {
   switch (tile_z) {
+    case 4: return launch_tz<4>(a, s);
     case 8: return launch_tz<8>(a, s);
     case 16: return launch_tz<16>(a, s);
     case 32: return launch_tz<32>(a, s);

@@ -52,12 +52,6 @@ __device__ __forceinline__ int xcd_linear_block(int b, int nb) {
   return x * q + min(x, r) + i;
 }
 
-// Stierstorfer weight for |q| <= 1: 1 below Q = 0.6, cos^2 ramp to 0 at 1.
-__device__ __forceinline__ float w_of_q(float q) {
-  const float u = fmaxf(fabsf(q) - FCT_WQ, 0.0f) * (1.5707963f / (1.0f - FCT_WQ));
-  const float c = __cosf(u);
-  return c * c;
-}
 
 template <int TZ, int PREC, int MODE>
 __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
@@ -168,7 +162,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
               hi *= rsv[qf + 1];
             }
             const float val = fmaf(wq, hi - lo, lo);
-            const float W = w_of_q(q);
+            const float W = fct_w_of_q(q);
             num[i] = fmaf(W, val, num[i]);
             den[i] += W;
           }
@@ -227,6 +221,7 @@ hipError_t launch_backproject(const BpArgs& a, int tile_z, int variant, int prec
   if (variant != FCT_BP_DIRECT && mode != kBpSelect)
     return launch_backproject_v1(a, tile_z, precision, mode, s);
   switch (tile_z) {
+    case 4: return launch_tz<4>(a, precision, mode, s);
     case 8: return launch_tz<8>(a, precision, mode, s);
     case 16: return launch_tz<16>(a, precision, mode, s);
     case 32: return launch_tz<32>(a, precision, mode, s);

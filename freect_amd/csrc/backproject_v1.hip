@@ -57,12 +57,6 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * q + min(x, r) + i;
 }
 
-__device__ __forceinline__ float w_of_q(float q) {
-  const float u = fmaxf(fabsf(q) - FCT_WQ, 0.0f) * (1.5707963f / (1.0f - FCT_WQ));
-  const float c = __cosf(u);
-  return c * c;
-}
-
 // Two adjacent taps (channel pf and pf+1) of one filtered row, as floats.
 template <int PREC>
 __device__ __forceinline__ float2 load_pair(__amdgpu_buffer_rsrc_t r, uint32_t el);
@@ -234,7 +228,7 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
           }
           const float wq = qi - qfl;
           const float val = fmaf(wq, hi - lo, lo);
-          const float W = sel ? w_of_q(q) : 0.0f;
+          const float W = sel ? fct_w_of_q(q) : 0.0f;
           num[i] = fmaf(W, val, num[i]);
           den[i] += W;
         }
@@ -297,6 +291,7 @@ hipError_t launch1(const BpArgs& a, int precision, int mode, hipStream_t s) {
 hipError_t launch_backproject_v1(const BpArgs& a, int tile_z, int precision, int mode,
                                  hipStream_t s) {
   switch (tile_z) {
+    case 4: return launch1<4>(a, precision, mode, s);
     case 8: return launch1<8>(a, precision, mode, s);
     case 16: return launch1<16>(a, precision, mode, s);
     case 32: return launch1<32>(a, precision, mode, s);

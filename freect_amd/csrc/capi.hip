@@ -124,8 +124,8 @@ int validate(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* s, const fct_recon_
   if (o) {
     if (o->precision < 0 || o->precision > 2)
       return set_err(ctx, "unknown precision %d", o->precision), FCT_ERR_INVALID_ARGUMENT;
-    if (o->tile_z != 0 && o->tile_z != 8 && o->tile_z != 16 && o->tile_z != 32)
-      return set_err(ctx, "tile_z must be 0, 8, 16 or 32"), FCT_ERR_INVALID_ARGUMENT;
+    if (o->tile_z != 0 && o->tile_z != 4 && o->tile_z != 8 && o->tile_z != 16 && o->tile_z != 32)
+      return set_err(ctx, "tile_z must be 0, 4, 8, 16 or 32"), FCT_ERR_INVALID_ARGUMENT;
     if (o->bp_variant < FCT_BP_AUTO || o->bp_variant > FCT_BP_LEAN)
       return set_err(ctx, "unknown bp_variant %d", o->bp_variant), FCT_ERR_INVALID_ARGUMENT;
   }
@@ -256,7 +256,7 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   p->g = fct_tables::bp_consts(p->ncols, p->nrows, s.channel_spacing, s.row_spacing, s.dsd, s.r,
                                s.central_channel, s.central_row, s.feed);
   p->scale = (float)(2.0 * kPi / (double)p->ppr);
-  p->tile_z = (opts && opts->tile_z) ? opts->tile_z : 16;
+  p->tile_z = (opts && opts->tile_z) ? opts->tile_z : 8;
   p->variant = (opts && opts->bp_variant) ? opts->bp_variant : FCT_BP_LEAN;
   if ((size_t)p->nviews * p->nrows * p->cpitch >= ((size_t)1 << 31))
     return fail((set_err(ctx, "filtered projections exceed 2^31 elements"), FCT_ERR_LIMIT));

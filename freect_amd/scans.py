@@ -168,7 +168,14 @@ def euler_zyz(phi, theta, psi):
         c, s = math.cos(a), math.sin(a)
         return np.array([[c, 0, s], [0, 1.0, 0], [-s, 0, c]])
 
-    return rz(phi) @ ry(theta) @ rz(psi)
+    def mul(A, B):  # same summation order as the C++ host (fct_host.cpp matmul3)
+        C = np.empty((3, 3))
+        for i in range(3):
+            for j in range(3):
+                C[i, j] = A[i, 0] * B[0, j] + A[i, 1] * B[1, j] + A[i, 2] * B[2, j]
+        return C
+
+    return mul(mul(rz(phi), ry(theta)), rz(psi))
 
 
 def make_phantom(seed: int, z_lo: float, z_hi: float, n: int = 20,

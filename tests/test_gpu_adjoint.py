@@ -164,7 +164,10 @@ def test_adjoint_fp64_finite_differences(ctx, oracle):
         an = np.vdot(dirn.astype(np.float64), g)
         worst = max(worst, abs(fd - an) / max(abs(fd), 1e-12))
     say("FP64 central differences vs GPU adjoint: worst rel %.3g over 64 directions" % worst)
-    assert worst < 1e-4
+    # The difference is <(A_32 - A_64) d, x>: the FP32 GPU operator against the
+    # FP64 one, about 1e-5 relative on images and larger on the random-sign
+    # inner products here.  A wrong transpose is off by O(1).
+    assert worst < 3e-4
 
 
 def test_autograd_wrappers(ctx):
