@@ -59,7 +59,7 @@ __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
     const float f = fmaf(x, cs.x, y * cs.y);
     const float kb = g.kappa * fct_asinf(p * g.invR);
     const float pidx_e = fmaf(p, g.inv_ds, g.cc), pidx_o = fmaf(-p, g.inv_ds, g.cc);
-    const float invL_e = 1.0f / ((sq - f) * g.tanc), invL_o = 1.0f / ((sq + f) * g.tanc);
+    const float invL_e = g.hr / ((sq - f) * g.tanc), invL_o = g.hr / ((sq + f) * g.tanc);
     float den[TZ];
 #pragma unroll
     for (int i = 0; i < TZ; ++i) den[i] = 0.0f;
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
         const float pidx = odd ? pidx_o : pidx_e;
         const float invL = odd ? invL_o : invL_e;
         const float ztv = ztc[v];
-        const float qoff = ((odd ? -kb : kb) - ztv) * invL;
+        const float row0 = fmaf((odd ? -kb : kb) - ztv, invL, g.cr);   // fct_bp_row0
         const float pfl = floorf(pidx);
         const bool chan_ok = active && pfl >= 0.0f && pfl <= (float)(g.ncols - 2);
         const int pf = chan_ok ? (int)pfl : 0;
@@ -85,17 +85,16 @@ __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
 #pragma unroll
         for (int i = 0; i < TZ; ++i) {
           if (i >= nz || !fct_bp_in_window(ztv, zs[i], g)) continue;
-          const float q = fct_bp_q(zs[i], invL, qoff);
-          const float qi = fct_bp_qidx(q, g);
+          const float qi = fmaf(zs[i], invL, row0);   // fct_bp_row
           const float qfl = floorf(qi);
           bool sel;
           if (CENTERED)
             sel = chan_ok && (uint32_t)(int)qfl <= (uint32_t)(g.nrows - 2);
           else
-            sel = chan_ok && !(q > 1.0f || q < -1.0f) && qfl >= 0.0f &&
+            sel = chan_ok && !(fabsf(qi - g.cr) > g.hr) && qfl >= 0.0f &&
                   qfl <= (float)(g.nrows - 2);
           if (!sel) continue;
-          const float W = fct_w_of_q(q);
+          const float W = fct_w_of_row(qi, g);
           if (pass == 0) {
             den[i] += W;
             continue;

@@ -114,8 +114,8 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
     const float kb = g.kappa * fct_asinf(p * g.invR);
     const float pidx_e = fmaf(p, g.inv_ds, g.cc);
     const float pidx_o = fmaf(-p, g.inv_ds, g.cc);
-    const float invL_e = 1.0f / ((sq - f) * g.tanc);
-    const float invL_o = 1.0f / ((sq + f) * g.tanc);
+    const float invL_e = g.hr / ((sq - f) * g.tanc);   // FctBpColumn::invLh
+    const float invL_o = g.hr / ((sq + f) * g.tanc);
 
     float num[TZ], den[TZ];
 #pragma unroll
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
       const float invL = odd ? invL_o : invL_e;
       const float kbs = odd ? -kb : kb;
       const float ztv = ztc[v];
-      const float qoff = (kbs - ztv) * invL;
+      const float row0 = fmaf(kbs - ztv, invL, g.cr);   // fct_bp_row0
       const float pfl = floorf(pidx);
       const bool chan_ok = active && pfl >= 0.0f && pfl <= (float)(g.ncols - 2);
       const int pf = chan_ok ? (int)pfl : 0;
@@ -140,10 +140,9 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
 #pragma unroll
       for (int i = 0; i < TZ; ++i) {
         if (i < nz && fct_bp_in_window(ztv, zs[i], g)) {
-          const float q = fct_bp_q(zs[i], invL, qoff);
-          const float qi = fct_bp_qidx(q, g);
+          const float qi = fmaf(zs[i], invL, row0);         // fct_bp_row
           const float qfl = floorf(qi);
-          const bool sel = chan_ok && !(q > 1.0f || q < -1.0f) && qfl >= 0.0f &&
+          const bool sel = chan_ok && !(fabsf(qi - g.cr) > g.hr) && qfl >= 0.0f &&
                            qfl <= (float)(g.nrows - 2);
           if (MODE == kBpSelect) {
             if (s0 + i == a.sel_is && ix == a.sel_ix && iy == a.sel_iy)
@@ -162,7 +161,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
               hi *= rsv[qf + 1];
             }
             const float val = fmaf(wq, hi - lo, lo);
-            const float W = fct_w_of_q(q);
+            const float W = fct_w_of_row(qi, g);
             num[i] = fmaf(W, val, num[i]);
             den[i] += W;
           }

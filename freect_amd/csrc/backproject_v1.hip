@@ -1,19 +1,22 @@
 // backproject_v1.hip -- instruction-lean helical WFBP backprojection.
 //
 // Same arithmetic and selection as backproject.hip (cuda_kernels.cuh:198-287
-// restated with the shared FP32 code of wfbp_math.h), reorganised to spend
-// fewer VALU instructions per selected (voxel, view) sample:
+// restated with the shared FP32 code of wfbp_math.h), organised to spend few
+// VALU instructions per selected (voxel, view) sample -- the kernel is
+// VALU-issue-bound:
 //   * per (theta~, view parity) each lane's z-interval of |q| <= 1 is
-//     [zt - kb - l tan, zt - kb + l tan]; a DPP wave reduction of its ends
-//     gives, per view, the slices any lane of the wave can select, and the
-//     other slices are skipped wave-uniformly (never changes a selection:
-//     the exact per-sample test still runs, with a one-slice margin);
-//   * when the central row is (nrows-1)/2, |q| <= 1 is implied exactly by the
-//     row test 0 <= floor(qidx) <= nrows-2 (fma is monotone), which becomes
-//     one unsigned compare;
-//   * taps are read through a buffer resource with 32-bit offsets (adjacent
-//     channels in one 8-byte load), out-of-range lanes read the 0 the buffer
-//     unit returns, and the lane's weight is zeroed instead of branching;
+//     [zt - kb - lt, zt - kb + lt]; a DPP wave reduction of its ends gives,
+//     per view, the slices any lane of the wave can select, intersected with
+//     the view window's exact slice range (a host table built with the same
+//     FP32 test); other slices are skipped wave-uniformly and the exact
+//     per-sample selection still runs, so selection is unchanged;
+//   * the detector row position is one fma per slice, and when the central
+//     row is (nrows-1)/2 the cone test |q| <= 1 is implied exactly by the row
+//     test, one unsigned compare;
+//   * taps are two 8-byte buffer loads with 32-bit byte offsets (the second
+//     row through the scalar offset); lanes that select nothing read whatever
+//     the range-checked buffer returns and get weight 0 instead of a branch;
+//   * FP16 taps are interpolated across channels with v_dot2_f32_f16;
 //   * the per-theta~ mean uses v_rcp.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -57,39 +60,66 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * q + min(x, r) + i;
 }
 
-// Two adjacent taps (channel pf and pf+1) of one filtered row, as floats.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// Per (lane, view) interpolation state across the two channels pf, pf+1.
 template <int PREC>
-__device__ __forceinline__ float2 load_pair(__amdgpu_buffer_rsrc_t r, uint32_t el);
+struct ChanInterp;
 
 template <>
-__device__ __forceinline__ float2 load_pair<FCT_PRECISION_FP32>(__amdgpu_buffer_rsrc_t r,
-                                                               uint32_t el) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(el * 4u), 0, 0);
-  return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
-}
+struct ChanInterp<FCT_PRECISION_FP32> {
+  float wp;
+  uint32_t base;   // byte offset of (view, row 0, pf)
+  __device__ void init(uint32_t view_bytes, uint32_t pf, float w) {
+    wp = w;
+    base = view_bytes + pf * 4u;
+  }
+  __device__ float row(__amdgpu_buffer_rsrc_t r, uint32_t off, int soff) const {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, soff, 0);
+    const float a = __uint_as_float(v[0]), b = __uint_as_float(v[1]);
+    return fmaf(wp, b - a, a);
+  }
+};
 
 template <>
-__device__ __forceinline__ float2 load_pair<FCT_PRECISION_FP16>(__amdgpu_buffer_rsrc_t r,
-                                                               uint32_t el) {
-  const uint32_t b = el * 2u;
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(b & ~3u), 0, 0);
-  const uint32_t w = __builtin_amdgcn_alignbyte(v[1], v[0], b & 2u);
-  const __half2 h = *reinterpret_cast<const __half2*>(&w);
-  return __half22float2(h);
-}
+struct ChanInterp<FCT_PRECISION_FP16> {
+  half2_t w2;      // (1 - wp, wp)
+  uint32_t base;   // 4-byte aligned byte offset of (view, row 0, pf & ~1)
+  uint32_t shift;  // 0 or 2: byte position of channel pf inside the loaded pair
+  __device__ void init(uint32_t view_bytes, uint32_t pf, float w) {
+    w2 = half2_t{(_Float16)(1.0f - w), (_Float16)w};
+    base = view_bytes + (pf & ~1u) * 2u;
+    shift = (pf & 1u) * 2u;
+  }
+  __device__ float row(__amdgpu_buffer_rsrc_t r, uint32_t off, int soff) const {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, soff, 0);
+    const uint32_t pair = __builtin_amdgcn_alignbyte(v[1], v[0], shift);
+    const half2_t h = __builtin_bit_cast(half2_t, pair);
+    return __builtin_amdgcn_fdot2(h, w2, 0.0f, false);
+  }
+};
 
 template <>
-__device__ __forceinline__ float2 load_pair<FCT_PRECISION_BF16>(__amdgpu_buffer_rsrc_t r,
-                                                               uint32_t el) {
-  const uint32_t b = el * 2u;
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(b & ~3u), 0, 0);
-  const uint32_t w = __builtin_amdgcn_alignbyte(v[1], v[0], b & 2u);
-  return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xFFFF0000u));
-}
+struct ChanInterp<FCT_PRECISION_BF16> {
+  float wp;
+  uint32_t base;
+  uint32_t shift;
+  __device__ void init(uint32_t view_bytes, uint32_t pf, float w) {
+    wp = w;
+    base = view_bytes + (pf & ~1u) * 2u;
+    shift = (pf & 1u) * 2u;
+  }
+  __device__ float row(__amdgpu_buffer_rsrc_t r, uint32_t off, int soff) const {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, soff, 0);
+    const uint32_t pair = __builtin_amdgcn_alignbyte(v[1], v[0], shift);
+    const float a = __uint_as_float(pair << 16), b = __uint_as_float(pair & 0xFFFF0000u);
+    return fmaf(wp, b - a, a);
+  }
+};
 
 template <int TZ, int PREC, int MODE, bool CENTERED>
 __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
-  constexpr int kElem = PREC == FCT_PRECISION_FP32 ? 4 : 2;
+  constexpr uint32_t kElem = PREC == FCT_PRECISION_FP32 ? 4u : 2u;
   const int lane = threadIdx.x;
   const int lin = xcd_block(blockIdx.x, gridDim.x);
   const int tiles = a.tiles_x * a.tiles_y;
@@ -122,18 +152,22 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
   const float dzs = nz > 1 ? (zlast - zfirst) / (float)(nz - 1) : 1.0f;
   const float inv_dzs = 1.0f / dzs;
   const float* ztc = a.zt + (size_t)chunk * a.nviews;
+  const int2* wrc = a.wrange + (size_t)chunk * a.nviews;
   const int2 vr = a.slab_vrange[slab];
   const int H = a.H;
   const uint32_t vstride = (uint32_t)g.nrows * (uint32_t)a.cpitch;
+  const uint32_t rowB = (uint32_t)a.cpitch * kElem;   // bytes per filtered row
 
   // Buffer resources based at the slab's first view (keeps offsets < 4 GiB).
   const char* fb = static_cast<const char*>(a.filt) + (size_t)vr.x * vstride * kElem;
-  const uint32_t fbytes = (uint32_t)min((size_t)(vr.y - vr.x) * vstride * kElem, (size_t)0xFFFFFFFFu);
-  const __amdgpu_buffer_rsrc_t rf = __builtin_amdgcn_make_buffer_rsrc((void*)fb, 0, fbytes, kRsrcWord3);
-  const bool scaled = PREC != FCT_PRECISION_FP32;
-  const void* sb = scaled ? (const void*)(a.row_scale + (size_t)vr.x * g.nrows) : (const void*)fb;
+  const uint32_t fbytes =
+      (uint32_t)min((size_t)(vr.y - vr.x) * vstride * kElem, (size_t)0xFFFFFFF0u);
+  const __amdgpu_buffer_rsrc_t rf =
+      __builtin_amdgcn_make_buffer_rsrc((void*)fb, 0, fbytes, kRsrcWord3);
+  constexpr bool kScaled = PREC != FCT_PRECISION_FP32;
+  const void* sb = kScaled ? (const void*)(a.row_scale + (size_t)vr.x * g.nrows) : (const void*)fb;
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)sb, 0, scaled ? (uint32_t)((vr.y - vr.x) * g.nrows * 4) : 0u, kRsrcWord3);
+      (void*)sb, 0, kScaled ? (uint32_t)((vr.y - vr.x) * g.nrows * 4) : 0u, kRsrcWord3);
 
   float acc[TZ];
   int cnt[TZ];
@@ -149,7 +183,7 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
     const int kmax = vr.y <= t ? 0 : (vr.y - t + H - 1) / H;
     if (kmin >= kmax) continue;
     const float2 cs = a.cs[t];
-    // fct_bp_column for (c, s) and (-c, -s).
+    // fct_bp_column for (c, s) and (-c, -s): p, sqrt and asin are shared.
     const float p = fmaf(x, cs.y, -(y * cs.x));
     const float sq = sqrtf(fmaf(-p, p, g.R2));
     const float f = fmaf(x, cs.x, y * cs.y);
@@ -167,68 +201,64 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
       int k = kmin + (((kmin & 1) != par) ? 1 : 0);
       if (k >= kmax) continue;
       const float pidx = par ? fmaf(-p, g.inv_ds, g.cc) : fmaf(p, g.inv_ds, g.cc);
-      const float Lt = (par ? sq + f : sq - f) * g.tanc;
-      const float invL = 1.0f / Lt;
+      const float lt = (par ? sq + f : sq - f) * g.tanc;
+      const float invLh = g.hr / lt;
       const float kbs = par ? -kb : kb;
       const float pfl = floorf(pidx);
       const bool chan_ok = active && pfl >= 0.0f && pfl <= (float)(g.ncols - 2);
       const uint32_t pf = chan_ok ? (uint32_t)(int)pfl : 0u;
       const float wp = pidx - pfl;
       // z-interval of |q| <= 1 relative to the view's table z, over the wave.
-      const float A = wave_min(active ? -kbs - Lt : BIG);
-      const float B = wave_max(active ? -kbs + Lt : -BIG);
+      const float A = wave_min(active ? -kbs - lt : BIG);
+      const float B = wave_max(active ? -kbs + lt : -BIG);
       for (; k < kmax; k += 2) {
         const int v = t + k * H;
         const float ztv = ztc[v];
-        const float qoff = (kbs - ztv) * invL;
-        // Slices any lane can select (window and q), one-slice margin.
-        const float za = ztv + fmaxf(A, -g.zrot), zb = ztv + fminf(B, g.zrot);
-        float fa = (za - zfirst) * inv_dzs, fbb = (zb - zfirst) * inv_dzs;
+        const int2 wr = wrc[v];   // exact window slice range (native indices)
+        float fa = (ztv + A - zfirst) * inv_dzs, fbb = (ztv + B - zfirst) * inv_dzs;
         if (fa > fbb) {
           const float tmp = fa;
           fa = fbb;
           fbb = tmp;
         }
-        const int ilo = max(0, (int)floorf(fa) - 1);
-        const int ihi = min(nz - 1, (int)ceilf(fbb) + 1);
+        const int ilo = max(max(0, (int)floorf(fa) - 1), wr.x - s0);
+        const int ihi = min(min(nz - 1, (int)ceilf(fbb) + 1), wr.y - s0);
         if (ilo > ihi) continue;
         ilo_t = min(ilo_t, ilo);
         ihi_t = max(ihi_t, ihi);
-        const uint32_t vbase = (uint32_t)(v - vr.x) * vstride + pf;
-        const uint32_t sbase = (uint32_t)(v - vr.x) * (uint32_t)g.nrows;
+        const float row0 = fmaf(kbs - ztv, invLh, g.cr);   // fct_bp_row0
+        ChanInterp<PREC> ci;
+        ci.init((uint32_t)(v - vr.x) * vstride * kElem, pf, wp);
+        const uint32_t sbase = (uint32_t)(v - vr.x) * (uint32_t)g.nrows * 4u;
 #pragma unroll
         for (int i = 0; i < TZ; ++i) {
           if (i < ilo || i > ihi) continue;
-          if (!fct_bp_in_window(ztv, zs[i], g)) continue;
-          const float q = fct_bp_q(zs[i], invL, qoff);
-          const float qi = fct_bp_qidx(q, g);
+          const float qi = fmaf(zs[i], invLh, row0);       // fct_bp_row
           const float qfl = floorf(qi);
           const int iq = (int)qfl;
           bool sel;
-          if (CENTERED) {
+          if (CENTERED)
             sel = chan_ok && (uint32_t)iq <= (uint32_t)(g.nrows - 2);
-          } else {
-            sel = chan_ok && !(q > 1.0f || q < -1.0f) && qfl >= 0.0f &&
+          else
+            sel = chan_ok && !(fabsf(qi - g.cr) > g.hr) && qfl >= 0.0f &&
                   qfl <= (float)(g.nrows - 2);
-          }
           if (MODE == kBpCount) {
             cnt[i] += sel ? 1 : 0;
             continue;
           }
-          const uint32_t el = sel ? vbase + (uint32_t)iq * (uint32_t)a.cpitch : 0xFFFFFFF0u / kElem;
-          float2 t0 = load_pair<PREC>(rf, el);
-          float2 t1 = load_pair<PREC>(rf, el + (uint32_t)a.cpitch);
-          float lo = fmaf(wp, t0.y - t0.x, t0.x);
-          float hi = fmaf(wp, t1.y - t1.x, t1.x);
-          if (PREC != FCT_PRECISION_FP32) {
-            const auto sc = __builtin_amdgcn_raw_buffer_load_b64(
-                rs, (int)((sel ? sbase + (uint32_t)iq : 0u) * 4u), 0, 0);
+          // 24-bit multiply: exact for selected rows (< 2^24); other lanes
+          // read an arbitrary in- or out-of-range offset and get weight 0.
+          const uint32_t off = ci.base + __umul24((uint32_t)iq, rowB);
+          float lo = ci.row(rf, off, 0);
+          float hi = ci.row(rf, off, (int)rowB);
+          if (kScaled) {
+            const auto sc = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(sbase + (uint32_t)iq * 4u),
+                                                                 0, 0);
             lo *= __uint_as_float(sc[0]);
             hi *= __uint_as_float(sc[1]);
           }
-          const float wq = qi - qfl;
-          const float val = fmaf(wq, hi - lo, lo);
-          const float W = sel ? fct_w_of_q(q) : 0.0f;
+          const float val = fmaf(qi - qfl, hi - lo, lo);
+          const float W = sel ? fct_w_of_row(qi, g) : 0.0f;
           num[i] = fmaf(W, val, num[i]);
           den[i] += W;
         }

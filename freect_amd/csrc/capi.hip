@@ -137,7 +137,7 @@ void free_plan_buffers(fct_wfbp_gpu_plan* p) {
                   p->d_slab_vrange, p->d_rebin,  p->d_Hn,        p->d_tw,       p->d_thk_off,
                   p->d_thk_idx,  p->d_thk_w,     p->d_filtered,  p->d_row_scale, p->d_native,
                   p->d_raw_stage, p->d_vol_stage, p->d_gbar, p->d_thkT_off, p->d_thkT_idx,
-                  p->d_thkT_w};
+                  p->d_thkT_w, p->d_wrange};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
 }
@@ -150,6 +150,7 @@ fct::BpArgs bp_args(fct_wfbp_gpu_plan* p) {
   a.zs = p->d_zs;
   a.zt = p->d_zt;
   a.slab_vrange = p->d_slab_vrange;
+  a.wrange = p->d_wrange;
   a.g = p->g;
   a.scale = p->scale;
   a.nx = p->recon.nx;
@@ -295,6 +296,29 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
     const double lo = std::floor(va) - 2, hi = std::ceil(vb) + 3;
     vr[sl].x = (int)std::max(0.0, std::min(lo, (double)p->nviews));
     vr[sl].y = (int)std::max(0.0, std::min(hi, (double)p->nviews));
+    // One buffer resource addresses a slab's views (32-bit byte offsets).
+    if ((size_t)(vr[sl].y - vr[sl].x) * p->nrows * p->cpitch * 4 >= ((size_t)1 << 32))
+      return fail((set_err(ctx, "a slab's view range exceeds 4 GiB of filtered rows"),
+                   FCT_ERR_LIMIT));
+  }
+
+  // Exact view-window slice ranges per (chunk, view): the slices s of the
+  // chunk with fct_bp_in_window(zt, zs) true form one interval (the test is
+  // monotone in zs); the same FP32 values and test as the kernels' per-sample
+  // form, evaluated once here.
+  std::vector<int2> wrange((size_t)p->nchunks * p->nviews);
+  for (int c = 0; c < p->nchunks; ++c) {
+    const int a0 = c * chunk, a1 = std::min(p->n_native, a0 + chunk);
+    for (int v = 0; v < p->nviews; ++v) {
+      int lo = p->n_native, hi = -1;
+      const float ztv = zt[(size_t)c * p->nviews + v];
+      for (int sidx = a0; sidx < a1; ++sidx)
+        if (fct_bp_in_window(ztv, zs[sidx], p->g)) {
+          lo = std::min(lo, sidx);
+          hi = std::max(hi, sidx);
+        }
+      wrange[(size_t)c * p->nviews + v] = make_int2(lo, hi);
+    }
   }
 
   // Rebinning table (cuda_kernels.cuh:17-36).
@@ -373,6 +397,7 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   PLAN_TRY(upload(&p->d_zs, zs, p->stream));
   PLAN_TRY(upload(&p->d_zt, zt, p->stream));
   PLAN_TRY(upload(&p->d_slab_vrange, vr, p->stream));
+  PLAN_TRY(upload(&p->d_wrange, wrange, p->stream));
   PLAN_TRY(upload(&p->d_rebin, tab, p->stream));
   PLAN_TRY(upload(&p->d_Hn, Hn, p->stream));
   PLAN_TRY(upload(&p->d_tw, tw, p->stream));

@@ -46,6 +46,7 @@ struct BpArgs {
   const float* zs;           // [n_native] slice z relative to its chunk reference
   const float* zt;           // [nchunks][nviews] view z relative to the chunk reference
   const int2* slab_vrange;   // [nslabs] views [x, y) a slab can select
+  const int2* wrange;        // [nchunks][nviews] native slices [x, y] inside the view window
   FctBpConsts g;
   float scale;               // 2 pi / ppr
   int32_t nx, ny, n_native, nviews, H, cpitch;
@@ -127,6 +128,7 @@ struct fct_wfbp_gpu_plan {
   float* d_zs = nullptr;
   float* d_zt = nullptr;
   int2* d_slab_vrange = nullptr;
+  int2* d_wrange = nullptr;
   fct::RebinDev* d_rebin = nullptr;
   float* d_Hn = nullptr;
   float2* d_tw = nullptr;

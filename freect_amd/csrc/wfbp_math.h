@@ -91,6 +91,7 @@ struct FctBpConsts {
   float cr;            // central row
   float zrot;          // |feed|: view window half width   (cuda_kernels.cuh:232)
   float half_fov_sq;   // (acq_fov/2)^2                    (recon_structs.h:72)
+  float wk;            // 1 / (hr (1 - Q)): weight ramp slope per row
   int32_t ncols;
   int32_t nrows;
 };
@@ -102,13 +103,18 @@ FCT_HD bool fct_in_fov(float x, float y, const FctBpConsts& g) {
 
 // Per (voxel column, parallel view) quantities.  (c,s) = (cos, sin) of the
 // parallel view angle; the complementary view (angle + pi) is (-c,-s).
+// The cone coordinate q = (z - zt + kb) / (l tan(cone/2)) of
+// cuda_kernels.cuh:239 enters only through the detector row position
+// qi = q * hr + cr (cuda_kernels.cuh:248 with an explicit central row), which
+// is computed directly: qi = z * invLh + (kb - zt) * invLh + cr.
 struct FctBpColumn {
-  float pidx;   // parallel channel position
-  float invL;   // 1/(l * tan(cone/2))
+  float pidx;   // parallel channel position               (cuda_kernels.cuh:246)
+  float lt;     // l * tan(cone/2): |q| <= 1 <=> |z - zt + kb| <= lt
+  float invLh;  // hr / (l * tan(cone/2)): rows per mm of z
   float kb;     // kappa * asin(p/R): source-z shift of this ray
 };
 
-// cuda_kernels.cuh:235-239, 246: p, l, q-denominator and channel position.
+// cuda_kernels.cuh:235-239, 246: p, l and the channel position.
 FCT_HD FctBpColumn fct_bp_column(float x, float y, float c, float s,
                                  const FctBpConsts& g) {
   FctBpColumn o;
@@ -116,37 +122,34 @@ FCT_HD FctBpColumn fct_bp_column(float x, float y, float c, float s,
   const float sq = sqrtf(fmaf(-p, p, g.R2));
   const float l = sq - fmaf(x, c, y * s);
   o.pidx = fmaf(p, g.inv_ds, g.cc);
-  o.invL = 1.0f / (l * g.tanc);
+  o.lt = l * g.tanc;
+  o.invLh = g.hr / o.lt;
   o.kb = g.kappa * fct_asinf(p * g.invR);
   return o;
 }
 
-// Per view: the offset term of q = (z - zt + kb) / (l tan).
-FCT_HD float fct_bp_qoffset(const FctBpColumn& col, float zt) {
-  return (col.kb - zt) * col.invL;
+// Per view: row position at z = 0 (slice z are offsets from a chunk
+// reference, like zt).
+FCT_HD float fct_bp_row0(const FctBpColumn& col, float zt, const FctBpConsts& g) {
+  return fmaf(col.kb - zt, col.invLh, g.cr);
 }
 
-// Per (column, view, slice): view window (cuda_kernels.cuh:232-233).
+// Per (column, view, slice): detector row position of the voxel.
+FCT_HD float fct_bp_row(float zs, const FctBpColumn& col, float row0) {
+  return fmaf(zs, col.invLh, row0);
+}
+
+// Per (view, slice): view window (cuda_kernels.cuh:232-233).
 FCT_HD bool fct_bp_in_window(float zt, float zs, const FctBpConsts& g) {
   return !(fabsf(zt - zs) > g.zrot);
 }
 
-// Per (column, view, slice): normalised cone coordinate q.
-FCT_HD float fct_bp_q(float zs, float invL, float qoff) {
-  return fmaf(zs, invL, qoff);
-}
-
-// Row position from q (cuda_kernels.cuh:248 with an explicit central row).
-FCT_HD float fct_bp_qidx(float q, const FctBpConsts& g) {
-  return fmaf(q, g.hr, g.cr);
-}
-
-// Selection of the 2x2 bilinear footprint.  A sample is used iff |q| <= 1
-// (cuda_kernels.cuh:243) and the whole 2x2 footprint lies on the detector.
-FCT_HD bool fct_bp_select(float q, float qidx, float pidx, const FctBpConsts& g,
-                          int* qf, int* pf) {
-  if (q > 1.0f || q < -1.0f) return false;
-  const float fq = floorf(qidx);
+// Selection of the 2x2 bilinear footprint: |q| <= 1 (cuda_kernels.cuh:243),
+// i.e. |qi - cr| <= hr, and the whole footprint on the detector.  When
+// cr == hr the row test alone implies the cone test (exactly, in FP32).
+FCT_HD bool fct_bp_select(float qi, float pidx, const FctBpConsts& g, int* qf, int* pf) {
+  if (fabsf(qi - g.cr) > g.hr) return false;
+  const float fq = floorf(qi);
   const float fp = floorf(pidx);
   if (!(fq >= 0.0f) || !(fp >= 0.0f)) return false;
   if (fq > (float)(g.nrows - 2) || fp > (float)(g.ncols - 2)) return false;
@@ -166,6 +169,20 @@ FCT_HD bool fct_bp_select(float q, float qidx, float pidx, const FctBpConsts& g,
 // cosine would not be.  Not part of index selection.
 FCT_HD float fct_w_of_q(float q) {
   const float t = fminf(fmaf(-fabsf(q), 1.0f / (1.0f - FCT_WQ), 1.0f / (1.0f - FCT_WQ)), 1.0f);
+  const float u = t * t;
+  float s = -3.5988433e-06f;
+  s = fmaf(s, u, 0.00016044118f);
+  s = fmaf(s, u, -0.004681754f);
+  s = fmaf(s, u, 0.079692625f);
+  s = fmaf(s, u, -0.6459641f);
+  s = fmaf(s, u, 1.5707964f);
+  return u * (s * s);
+}
+
+// The same weight from the row position: |q| = |qi - cr| / hr, so
+// t = min(1/(1-Q) - |qi - cr| * wk, 1) with wk = 1 / (hr (1-Q)).
+FCT_HD float fct_w_of_row(float qi, const FctBpConsts& g) {
+  const float t = fminf(fmaf(-fabsf(qi - g.cr), g.wk, 1.0f / (1.0f - FCT_WQ)), 1.0f);
   const float u = t * t;
   float s = -3.5988433e-06f;
   s = fmaf(s, u, 0.00016044118f);
@@ -260,6 +277,7 @@ inline FctBpConsts bp_consts(int ncols, int nrows, double dch, double drow,
   g.cr = (float)cr;
   g.zrot = (float)std::fabs(feed);
   g.half_fov_sq = (float)(0.25 * acq * acq);
+  g.wk = (float)(1.0 / (hr * (1.0 - 0.6)));
   g.ncols = ncols;
   g.nrows = nrows;
   return g;

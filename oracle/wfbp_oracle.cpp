@@ -67,16 +67,6 @@ double w_of_q(double q) {
   return 0.0;
 }
 
-float w_of_q_f32(float q) {
-  q = fabsf(q);
-  if (q < FCT_WQ) return 1.0f;
-  if (q < 1.0f) {
-    const float c = cosf((0.5f * 3.14159265f) * (q - FCT_WQ) / (1.0f - FCT_WQ));
-    return c * c;
-  }
-  return 0.0f;
-}
-
 void fft_inplace(std::complex<double>* a, int n, bool inverse) {
   for (int i = 1, j = 0; i < n; ++i) {
     int bit = n >> 1;
@@ -197,10 +187,9 @@ int bp_core(const or_scan* s, const or_recon* r, const P* filtered, int v0,
           for (int i = 0; i < ns; ++i) {
             const float zt = zt32[(size_t)((s0 + i) / chunk - c_lo) * G.nviews + v];
             if (!fct_bp_in_window(zt, zs32[i], g)) continue;
-            const float q32 = fct_bp_q(zs32[i], cl.invL, fct_bp_qoffset(cl, zt));
-            const float qi32 = fct_bp_qidx(q32, g);
+            const float qi32 = fct_bp_row(zs32[i], cl, fct_bp_row0(cl, zt, g));
             int qf, pf;
-            if (!fct_bp_select(q32, qi32, cl.pidx, g, &qf, &pf)) continue;
+            if (!fct_bp_select(qi32, cl.pidx, g, &qf, &pf)) continue;
             T wq, wp, W;
             if (sizeof(T) == sizeof(double)) {
               const double q64 = (sg.zn[s0 + i] - zsrc) / (l * G.tanc);
@@ -211,7 +200,7 @@ int bp_core(const or_scan* s, const or_recon* r, const P* filtered, int v0,
             } else {
               wq = (T)(qi32 - (float)qf);
               wp = (T)(cl.pidx - (float)pf);
-              W = (T)w_of_q_f32(q32);
+              W = (T)fct_w_of_row(qi32, g);
             }
             T val = (T)0;
             if (view) {
@@ -435,9 +424,9 @@ int or_bp_selection(const or_scan* s, const or_recon* r, int ix, int iy, int is,
     const FctBpColumn cl = fct_bp_column(x, y, sg32 * c, sg32 * sn, g);
     const float zt = fct_tables::view_z_rel(s->z0, G.dz, v, zref);
     if (!fct_bp_in_window(zt, zs, g)) continue;
-    const float q = fct_bp_q(zs, cl.invL, fct_bp_qoffset(cl, zt));
+    const float qi = fct_bp_row(zs, cl, fct_bp_row0(cl, zt, g));
     int qf, pf;
-    if (fct_bp_select(q, fct_bp_qidx(q, g), cl.pidx, g, &qf, &pf)) sel[v] = qf * 65536 + pf;
+    if (fct_bp_select(qi, cl.pidx, g, &qf, &pf)) sel[v] = qf * 65536 + pf;
   }
   return 0;
 }

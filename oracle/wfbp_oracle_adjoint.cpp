@@ -103,9 +103,9 @@ int or_backproject_adjoint(const or_scan* s, const or_recon* r, const double* xb
           const double zref = zn[((s0 + i) / chunk) * chunk];
           const float zt = fct_tables::view_z_rel(s->z0, dz, v, zref);
           if (!fct_bp_in_window(zt, zs32[i], g)) continue;
-          const float q32 = fct_bp_q(zs32[i], cl.invL, fct_bp_qoffset(cl, zt));
+          const float qi32 = fct_bp_row(zs32[i], cl, fct_bp_row0(cl, zt, g));
           int qf, pf;
-          if (!fct_bp_select(q32, fct_bp_qidx(q32, g), cl.pidx, g, &qf, &pf)) continue;
+          if (!fct_bp_select(qi32, cl.pidx, g, &qf, &pf)) continue;
           const double q64 = (zn[s0 + i] - zsrc) / (l * tanc);
           const double W = w_of_q(q64);
           smp.push_back({v, i, qf, pf, q64 * hr + s->central_row - qf, pidx64 - pf, W});
