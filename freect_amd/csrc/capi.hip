@@ -219,6 +219,9 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   p->recon = *recon;
   if (opts) p->opts = *opts;
   p->stream = static_cast<hipStream_t>(stream);   // NULL: the null stream
+  if (hipDeviceGetAttribute(&p->ncu, hipDeviceAttributeMultiprocessorCount, p->device) !=
+      hipSuccess)
+    p->ncu = 1;
   auto fail = [&](int code) {
     free_plan_buffers(p);
     delete p;
@@ -468,6 +471,7 @@ FCT_API int fct_wfbp_gpu_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, vo
   a.log2n = p->log2n;
   a.precision = p->opts.precision;
   a.nlines = p->nviews * p->nrows;
+  a.ncu = p->ncu;
   HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
   return FCT_OK;
 }
@@ -519,6 +523,7 @@ FCT_API int fct_wfbp_gpu_filter(fct_wfbp_gpu_plan* p, const float* lines_in, flo
   a.log2n = p->log2n;
   a.precision = FCT_PRECISION_FP32;
   a.nlines = p->nviews * p->nrows;
+  a.ncu = p->ncu;
   HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
   return FCT_OK;
 }

@@ -17,6 +17,7 @@
 // Storage: FP32, or FP16/BF16 with a power-of-two scale per line chosen so
 // that max|y|/scale <= 2^15.
 #include <hip/hip_bf16.h>
+#include <algorithm>
 #include <hip/hip_fp16.h>
 
 #include "fft_lds.h"
@@ -168,9 +169,204 @@ __global__ __launch_bounds__(kRfThreads) void rebin_filter_kernel(RebinFilterArg
   }
 }
 
+// Register-resident variant for N in {512, 1024, 2048, 4096}: N / 8
+// threads, 8 values each (fft_lds.h).  The block is persistent over line
+// pairs, so the filter spectrum at its frequencies and its twiddles are
+// loaded once into registers.  Thread t owns
+// channels and frequencies t + m * N / 8: the rebinned values enter the first
+// pass without an exchange, the spectrum product sits between the two
+// transforms without an exchange, and the last inverse pass writes the
+// filtered channels directly (coalesced).  Lines have ncols <= N / 2
+// channels, so registers m >= 4 start as zero.
+template <int N, int P>
+__device__ __forceinline__ void load_twiddles(FctCplx* out, const FctCplx* __restrict__ tw, int t) {
+  constexpr int R = FctFftShape<N>::radix(P), Ns = FctFftShape<N>::span(P);
+#pragma unroll
+  for (int u = 0; u < 8 / R; ++u)
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      out[u * R + r] = r ? tw[fct_fft_tw_index<N>(t + u * FctFftShape<N>::T, r, R, Ns)]
+                         : FctCplx{1.0f, 0.0f};
+}
+
+// Exchange ahead of pass P (P >= 1) through LDS.
+template <int N, int P, bool kPingPong>
+__device__ __forceinline__ void fft_exchange(FctCplx* v, FctCplx* lds, int t) {
+  constexpr int kSlots = N + N / 8;
+  FctCplx* buf = kPingPong ? lds + (P & 1) * kSlots : lds;
+  if (!kPingPong) __syncthreads();   // earlier loads from the single buffer are done
+  fct_fft_pass_store<N, P - 1>(v, buf, t);
+  __syncthreads();
+  fct_fft_pass_load<N, P>(v, buf, t);
+}
+
+// One forward transform of the 8 values per thread in v (natural-order
+// inputs t + m T in v[m]; outputs t + m T in v[fct_fft_final_reg<N>(m)]).
+template <int N, bool kPingPong>
+__device__ __forceinline__ void fft_forward_regs(FctCplx* v, FctCplx* lds, int t,
+                                                 const FctCplx* tw1, const FctCplx* tw2,
+                                                 const FctCplx* tw3) {
+  fct_fft_pass_compute<N, 0>(v, nullptr);
+  fft_exchange<N, 1, kPingPong>(v, lds, t);
+  fct_fft_pass_compute<N, 1>(v, tw1);
+  fft_exchange<N, 2, kPingPong>(v, lds, t);
+  fct_fft_pass_compute<N, 2>(v, tw2);
+  if constexpr (FctFftShape<N>::kPasses == 4) {
+    fft_exchange<N, 3, kPingPong>(v, lds, t);
+    fct_fft_pass_compute<N, 3>(v, tw3);
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs a) {
+  using S = FctFftShape<N>;
+  constexpr int T = S::T;
+  constexpr bool kPingPong = N <= 2048;
+  constexpr int kSlots = N + N / 8;
+  __shared__ FctCplx lds[kPingPong ? 2 * kSlots : kSlots];
+  __shared__ float red[2][T / 64];
+  const int t = threadIdx.x;
+  const FctCplx* tw = reinterpret_cast<const FctCplx*>(a.tw);
+
+  FctCplx tw1[8], tw2[8], tw3[8];
+  load_twiddles<N, 1>(tw1, tw, t);
+  load_twiddles<N, 2>(tw2, tw, t);
+  if constexpr (S::kPasses == 4) load_twiddles<N, 3>(tw3, tw, t);
+  float hn[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) hn[m] = a.Hn[t + m * T];
+
+  const int npairs = (a.nlines + 1) >> 1;
+  for (int pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
+    const int l0 = 2 * pair, l1 = l0 + 1;
+    const bool has1 = l1 < a.nlines;
+    const int j0 = l0 / a.nrows, r0 = l0 - j0 * a.nrows;
+    const int j1 = has1 ? l1 / a.nrows : 0, r1 = has1 ? l1 - j1 * a.nrows : 0;
+    FctCplx v[8];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int c = t + m * T;
+      float va = 0.0f, vb = 0.0f;
+      if (a.lines_in) {
+        if (c < a.ncols) {
+          va = a.lines_in[(size_t)l0 * a.cpitch + c];
+          if (has1) vb = a.lines_in[(size_t)l1 * a.cpitch + c];
+        }
+      } else if (c < a.ncols) {
+        const RebinDev e = a.tab[c];   // small table, cache resident
+        if (e.valid) {   // fan channel outside [0, ncols): 0 (cuda_kernels.cuh:32-34)
+          va = rebin_sample(a.raw, j0, r0, e, a.nviews, a.nrows, a.ncols);
+          if (has1) vb = rebin_sample(a.raw, j1, r1, e, a.nviews, a.nrows, a.ncols);
+        }
+      }
+      v[m] = FctCplx{va, vb};
+      v[m + 4] = FctCplx{0.0f, 0.0f};
+    }
+    fft_forward_regs<N, kPingPong>(v, lds, t, tw1, tw2, tw3);
+    FctCplx w[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const FctCplx z = v[fct_fft_final_reg<N>(m)];
+      w[m] = FctCplx{z.x * hn[m], -z.y * hn[m]};   // conj(H/N * Z)
+    }
+    fft_forward_regs<N, kPingPong>(w, lds, t, tw1, tw2, tw3);   // y = conj(result)
+
+    const size_t o0 = (size_t)l0 * a.cpitch, o1 = (size_t)l1 * a.cpitch;
+    if (a.precision == FCT_PRECISION_FP32) {
+      float* out = static_cast<float*>(a.out);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int c = t + m * T;
+        if (c < a.cpitch) {
+          const FctCplx y = w[fct_fft_final_reg<N>(m)];
+          const bool in = c < a.ncols;
+          out[o0 + c] = in ? y.x : 0.0f;
+          if (has1) out[o1 + c] = in ? -y.y : 0.0f;
+        }
+      }
+      continue;
+    }
+    // Per-line power-of-two scale: max|y| / scale <= 2^15.
+    float m0 = 0.0f, m1 = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if (t + m * T < a.ncols) {
+        const FctCplx y = w[fct_fft_final_reg<N>(m)];
+        m0 = fmaxf(m0, fabsf(y.x));
+        m1 = fmaxf(m1, fabsf(y.y));
+      }
+    }
+    m0 = wave_max(m0);
+    m1 = wave_max(m1);
+    if ((t & 63) == 0) {
+      red[0][t >> 6] = m0;
+      red[1][t >> 6] = m1;
+    }
+    __syncthreads();
+    m0 = red[0][0];
+    m1 = red[1][0];
+#pragma unroll
+    for (int i = 1; i < T / 64; ++i) {
+      m0 = fmaxf(m0, red[0][i]);
+      m1 = fmaxf(m1, red[1][i]);
+    }
+    int e0 = 0, e1 = 0;
+    frexpf(m0, &e0);
+    frexpf(m1, &e1);
+    const float s0 = m0 > 0.0f ? ldexpf(1.0f, e0 - 15) : 1.0f;
+    const float s1 = m1 > 0.0f ? ldexpf(1.0f, e1 - 15) : 1.0f;
+    const float i0 = 1.0f / s0, i1 = 1.0f / s1;   // exact: powers of two
+    if (t == 0) {
+      a.row_scale[l0] = s0;
+      if (has1) a.row_scale[l1] = s1;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int c = t + m * T;
+      if (c >= a.cpitch) continue;
+      const FctCplx y = w[fct_fft_final_reg<N>(m)];
+      const bool in = c < a.ncols;
+      const float y0 = in ? y.x * i0 : 0.0f, y1 = in ? -y.y * i1 : 0.0f;
+      if (a.precision == FCT_PRECISION_FP16) {
+        __half* out = static_cast<__half*>(a.out);
+        out[o0 + c] = __float2half_rn(y0);
+        if (has1) out[o1 + c] = __float2half_rn(y1);
+      } else {
+        __hip_bfloat16* out = static_cast<__hip_bfloat16*>(a.out);
+        out[o0 + c] = __float2bfloat16(y0);
+        if (has1) out[o1 + c] = __float2bfloat16(y1);
+      }
+    }
+  }
+}
+
+template <int N>
+hipError_t launch_reg(const RebinFilterArgs& a, hipStream_t s) {
+  static int per_cu = -1;   // resident blocks per CU (same for every device here)
+  if (per_cu < 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rebin_filter_reg_kernel<N>, N / 8, 0) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    per_cu = nb;
+  }
+  const int npairs = (a.nlines + 1) / 2;
+  const int blocks = std::max(1, std::min(npairs, per_cu * std::max(a.ncu, 1)));
+  hipLaunchKernelGGL(rebin_filter_reg_kernel<N>, dim3(blocks), dim3(N / 8), 0, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s) {
+  if (a.nlines <= 0) return hipSuccess;
+  switch (a.nfft) {
+    case 512: return launch_reg<512>(a, s);
+    case 1024: return launch_reg<1024>(a, s);
+    case 2048: return launch_reg<2048>(a, s);
+    case 4096: return launch_reg<4096>(a, s);
+  }
   const int blocks = (a.nlines + 1) / 2;
   const size_t lds = 2 * (size_t)a.nfft * sizeof(FctCplx);
   hipLaunchKernelGGL(rebin_filter_kernel, dim3(blocks), dim3(kRfThreads), lds, s, a);

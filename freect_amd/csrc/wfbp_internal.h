@@ -30,6 +30,7 @@ struct RebinFilterArgs {
   int32_t precision;
   int32_t nlines;            // nviews * nrows
   const float* lines_in;     // filter-only mode: [line][cpitch] FP32 input (no rebinning)
+  int32_t ncu;               // compute units of the device (persistent grid size)
 };
 
 // Everything the backprojection kernel reads.
@@ -117,6 +118,7 @@ struct fct_wfbp_gpu_plan {
 
   int ncols = 0, nrows = 0, nviews = 0, ppr = 0, H = 0, nfft = 0, log2n = 0, cpitch = 0;
   int n_native = 0, n_req = 0, nchunks = 0, tile_z = 16, variant = 0, nslabs = 0;
+  int ncu = 0;                      // compute units of the plan's device
   double dgamma = 0, ds = 0, dtheta = 0, dz = 0, csw = 0, acq = 0;
   FctBpConsts g{};
   float scale = 0;

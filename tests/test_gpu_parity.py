@@ -84,6 +84,49 @@ def test_rebin_filter_matches_oracle(ctx, oracle, filt):
     assert err < tol
 
 
+@pytest.mark.parametrize("n_channels", [96, 200, 400, 888, 1500])
+def test_rebin_filter_fft_sizes(ctx, oracle, n_channels):
+    # nfft 256 (generic LDS kernel) and 512 / 1024 / 2048 / 4096 (the
+    # register-resident kernel, one instantiation each), FP32 against the
+    # FP64 oracle, then FP16 / BF16 storage against the FP32 output: per line
+    # the stored value times its power-of-two scale is within half an ulp of
+    # the format at the line maximum (2^-11 / 2^-8 relative) plus the FFT
+    # round-off.
+    w = S.tiny(n_channels=n_channels, n_rows=4, ppr=96, rotations=2.0)
+    sc, rc = w.scan, w.recon
+    raw_t = _raw_gpu(ctx, w, 0.0)
+    raw = to_np(raw_t)
+    d = oracle.geometry(sc, rc)
+    say("rebin_filter sizes", n_channels, "nfft", d.nfft)
+    x = oracle.rebin(sc, raw)
+    h = oracle.filter_kernel(d.nfft, d.ds, rc.filter_kind, rc.filter_c, rc.filter_a)
+    ref = oracle.filter_rows(x, h, d.nfft, use_fft=n_channels > 400)
+    Hk = np.real(np.fft.fft(np.fft.ifftshift(h)))
+    tol = 1e-7 * math.log2(d.nfft) * np.abs(Hk).max() * np.abs(x).max()
+    f32 = None
+    for prec, rel in (("fp32", 0.0), ("fp16", 2.0 ** -11), ("bf16", 2.0 ** -8)):
+        p = _plan(ctx, w, precision=prec)
+        filt_t, scale_t = p.alloc_filtered()
+        p.rebin_filter(raw_t, filt_t, scale_t)
+        p.synchronize()
+        full_rows = to_np(p.filtered_view(filt_t).float())
+        assert np.all(full_rows[..., sc.n_channels:] == 0.0)   # pitch padding is zero
+        got = full_rows[..., : sc.n_channels]
+        if prec == "fp32":
+            err = np.max(np.abs(got - ref))
+            say("  fp32 err %.3g tol %.3g" % (err, tol))
+            assert err < tol
+            f32 = got
+            continue
+        s = to_np(scale_t).reshape(sc.n_views, sc.n_rows, 1)
+        deq = got * s
+        line_max = np.abs(f32).max(axis=-1, keepdims=True)
+        assert np.all(line_max <= s * 2.0 ** 15 + 1e-30)
+        err = np.abs(deq - f32) - (rel * line_max + tol)
+        say("  %s worst margin %.3g" % (prec, err.max()))
+        assert np.all(err <= 0.0)
+
+
 @pytest.mark.parametrize("cr_shift", [0.0, 0.37])
 def test_selection_bit_exact_counts(ctx, oracle, cr_shift):
     # cr_shift != 0 moves the central row off (nrows-1)/2 and exercises the
