@@ -62,6 +62,14 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
+// Read-only table entry at a wave-uniform address, loaded through the
+// constant address space so it becomes a scalar load into SGPRs even where
+// the kernel also stores to global memory (the persistent batch loop).
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // Per (lane, view) interpolation state across the two channels pf, pf+1.
 template <int PREC>
 struct ChanInterp;
@@ -117,11 +125,11 @@ struct ChanInterp<FCT_PRECISION_BF16> {
   }
 };
 
+// One wave reconstructs tile `lin` (slab-major order) of the job described by a.
 template <int TZ, int PREC, int MODE, bool CENTERED>
-__global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
+__device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   constexpr uint32_t kElem = PREC == FCT_PRECISION_FP32 ? 4u : 2u;
   const int lane = threadIdx.x;
-  const int lin = xcd_block(blockIdx.x, gridDim.x);
   const int tiles = a.tiles_x * a.tiles_y;
   const int slab = lin / tiles;
   const int tile = lin - slab * tiles;
@@ -146,14 +154,15 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
   const int chunk = s0 / fct_tables::kSliceChunk;
   float zs[TZ];
 #pragma unroll
-  for (int i = 0; i < TZ; ++i) zs[i] = a.zs[s0 + (i < nz ? i : nz - 1)];
+  for (int i = 0; i < TZ; ++i) zs[i] = ld_uniform(a.zs + s0 + (i < nz ? i : nz - 1));
   // Slice spacing inside the slab (uniform, FP32; used only for skipping).
   const float zfirst = zs[0], zlast = zs[nz - 1];
   const float dzs = nz > 1 ? (zlast - zfirst) / (float)(nz - 1) : 1.0f;
   const float inv_dzs = 1.0f / dzs;
   const float* ztc = a.zt + (size_t)chunk * a.nviews;
-  const int2* wrc = a.wrange + (size_t)chunk * a.nviews;
-  const int2 vr = a.slab_vrange[slab];
+  const int* wrc = reinterpret_cast<const int*>(a.wrange + (size_t)chunk * a.nviews);
+  const int* vrp = reinterpret_cast<const int*>(a.slab_vrange + slab);
+  const int2 vr = make_int2(ld_uniform(vrp), ld_uniform(vrp + 1));
   const int H = a.H;
   const uint32_t vstride = (uint32_t)g.nrows * (uint32_t)a.cpitch;
   const uint32_t rowB = (uint32_t)a.cpitch * kElem;   // bytes per filtered row
@@ -182,7 +191,8 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
     const int kmin = vr.x <= t ? 0 : (vr.x - t + H - 1) / H;
     const int kmax = vr.y <= t ? 0 : (vr.y - t + H - 1) / H;
     if (kmin >= kmax) continue;
-    const float2 cs = a.cs[t];
+    const float* csp = reinterpret_cast<const float*>(a.cs + t);
+    const float2 cs = make_float2(ld_uniform(csp), ld_uniform(csp + 1));
     // fct_bp_column for (c, s) and (-c, -s): p, sqrt and asin are shared.
     const float p = fmaf(x, cs.y, -(y * cs.x));
     const float sq = sqrtf(fmaf(-p, p, g.R2));
@@ -213,8 +223,9 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
       const float B = wave_max(active ? -kbs + lt : -BIG);
       for (; k < kmax; k += 2) {
         const int v = t + k * H;
-        const float ztv = ztc[v];
-        const int2 wr = wrc[v];   // exact window slice range (native indices)
+        const float ztv = ld_uniform(ztc + v);
+        // Exact window slice range (native indices).
+        const int2 wr = make_int2(ld_uniform(wrc + 2 * v), ld_uniform(wrc + 2 * v + 1));
         float fa = (ztv + A - zfirst) * inv_dzs, fbb = (ztv + B - zfirst) * inv_dzs;
         if (fa > fbb) {
           const float tmp = fa;
@@ -288,6 +299,42 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
   }
 }
 
+template <int TZ, int PREC, int MODE, bool CENTERED>
+__global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
+  bp_v1_tile<TZ, PREC, MODE, CENTERED>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// Persistent backprojection of many jobs in one launch.  The tiles of all
+// jobs form one list (job-major, slab-major inside a job) split into eight
+// contiguous ranges, one per XCD (workgroup b runs on XCD b mod 8), so that
+// neighbouring tiles -- which read the same projection rows -- share an L2.
+// A wave takes the next tile of its XCD's range with one atomic and moves on
+// to the other ranges when its own is exhausted; every wave exits once all
+// eight counters have passed their range ends.  A tile is computed by the
+// same code as in bp_v1_kernel, so each job's volume is bitwise identical to
+// its own single-job launch with the same tile_z.
+template <int TZ, int PREC, bool CENTERED>
+__global__ __launch_bounds__(64) void bp_batch_kernel(BpBatchArgs b,
+                                                      const BpArgs* __restrict__ jobs,
+                                                      const int32_t* __restrict__ tile_start) {
+  const int xcd = blockIdx.x & 7;
+  for (int step = 0; step < 8; ++step) {
+    const int r = (xcd + step) & 7;
+    const int lo = (int)(((long long)b.total_tiles * r) >> 3);
+    const int hi = (int)(((long long)b.total_tiles * (r + 1)) >> 3);
+    while (true) {
+      uint32_t got = 0;
+      if (threadIdx.x == 0) got = atomicAdd(&b.queue[r], 1u);
+      const int tile = lo + (int)__builtin_amdgcn_readfirstlane(__shfl(got, 0, 64));
+      if (tile >= hi) break;
+      int j = 0;   // job: last j with tile_start[j] <= tile
+      for (int span = 32; span > 0; span >>= 1)
+        if (j + span < b.n_jobs && ld_uniform(tile_start + j + span) <= tile) j += span;
+      bp_v1_tile<TZ, PREC, kBpAccumulate, CENTERED>(jobs[j], tile - ld_uniform(tile_start + j));
+    }
+  }
+}
+
 template <int TZ, int PREC, bool C>
 hipError_t launch3(const BpArgs& a, int mode, hipStream_t s) {
   const int blocks = a.tiles_x * a.tiles_y * a.nslabs;
@@ -316,7 +363,64 @@ hipError_t launch1(const BpArgs& a, int precision, int mode, hipStream_t s) {
   return hipErrorInvalidValue;
 }
 
+template <int TZ, int PREC>
+hipError_t launch_batch2(const BpBatchArgs& b, bool centered, int grid, hipStream_t s) {
+  if (centered)
+    hipLaunchKernelGGL((bp_batch_kernel<TZ, PREC, true>), dim3(grid), dim3(64), 0, s, b, b.jobs,
+                       b.tile_start);
+  else
+    hipLaunchKernelGGL((bp_batch_kernel<TZ, PREC, false>), dim3(grid), dim3(64), 0, s, b, b.jobs,
+                       b.tile_start);
+  return hipGetLastError();
+}
+
+template <int TZ>
+hipError_t launch_batch1(const BpBatchArgs& b, int precision, bool centered, int grid,
+                         hipStream_t s) {
+  switch (precision) {
+    case FCT_PRECISION_FP32: return launch_batch2<TZ, FCT_PRECISION_FP32>(b, centered, grid, s);
+    case FCT_PRECISION_FP16: return launch_batch2<TZ, FCT_PRECISION_FP16>(b, centered, grid, s);
+    case FCT_PRECISION_BF16: return launch_batch2<TZ, FCT_PRECISION_BF16>(b, centered, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 }  // namespace
+
+hipError_t launch_backproject_batch(const BpBatchArgs& b, int tile_z, int precision,
+                                    bool centered, int grid, hipStream_t s) {
+  if (b.n_jobs < 1 || b.n_jobs > 64 || grid < 8 || (grid & 7)) return hipErrorInvalidValue;
+  switch (tile_z) {
+    case 4: return launch_batch1<4>(b, precision, centered, grid, s);
+    case 8: return launch_batch1<8>(b, precision, centered, grid, s);
+    case 16: return launch_batch1<16>(b, precision, centered, grid, s);
+    case 32: return launch_batch1<32>(b, precision, centered, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t backproject_batch_waves_per_cu(int tile_z, int precision, int* waves) {
+  // Occupancy of the FP32 / FP16 / BF16 instantiations (the centred and
+  // general forms use the same registers to within a few).
+  const void* f = nullptr;
+#define FCT_BATCH_K(TZ)                                                             \
+  case TZ:                                                                          \
+    f = precision == FCT_PRECISION_FP32                                             \
+            ? (const void*)bp_batch_kernel<TZ, FCT_PRECISION_FP32, false>           \
+            : precision == FCT_PRECISION_FP16                                       \
+                  ? (const void*)bp_batch_kernel<TZ, FCT_PRECISION_FP16, false>     \
+                  : (const void*)bp_batch_kernel<TZ, FCT_PRECISION_BF16, false>;    \
+    break;
+  switch (tile_z) {
+    FCT_BATCH_K(4)
+    FCT_BATCH_K(8)
+    FCT_BATCH_K(16)
+    FCT_BATCH_K(32)
+    default: return hipErrorInvalidValue;
+  }
+#undef FCT_BATCH_K
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(waves, f, 64, 0);
+}
 
 hipError_t launch_backproject_v1(const BpArgs& a, int tile_z, int precision, int mode,
                                  hipStream_t s) {

@@ -142,8 +142,58 @@ void free_plan_buffers(fct_wfbp_gpu_plan* p) {
     if (b) (void)hipFree(b);
 }
 
-fct::BpArgs bp_args(fct_wfbp_gpu_plan* p) {
-  fct::BpArgs a{};
+// Views a slab of tz slices can select: table z within one feed of the
+// slab's slices (a superset of the view window), with a margin of views.
+// False if one slab's views exceed the 4 GiB a buffer resource addresses.
+bool slab_view_ranges(const fct_wfbp_gpu_plan* p, const std::vector<double>& zn, int tz,
+                      std::vector<int2>* out) {
+  const fct_scan_geometry& s = p->scan;
+  const int nslabs = (p->n_native + tz - 1) / tz;
+  std::vector<int2>& vr = *out;
+  vr.assign(nslabs, make_int2(0, 0));
+  for (int sl = 0; sl < nslabs; ++sl) {
+    const int a0 = sl * tz, a1 = std::min(p->n_native, a0 + tz) - 1;
+    const double zlo = std::min(zn[a0], zn[a1]) - std::fabs(s.feed);
+    const double zhi = std::max(zn[a0], zn[a1]) + std::fabs(s.feed);
+    double va = (zlo - s.z0) / p->dz, vb = (zhi - s.z0) / p->dz;
+    if (va > vb) std::swap(va, vb);
+    const double lo = std::floor(va) - 2, hi = std::ceil(vb) + 3;
+    vr[sl].x = (int)std::max(0.0, std::min(lo, (double)p->nviews));
+    vr[sl].y = (int)std::max(0.0, std::min(hi, (double)p->nviews));
+    if ((size_t)(vr[sl].y - vr[sl].x) * p->nrows * p->cpitch * 4 >= ((size_t)1 << 32))
+      return false;
+  }
+  return true;
+}
+
+std::vector<double> native_slice_z(const fct_wfbp_gpu_plan* p) {
+  std::vector<double> zn(p->n_native);
+  for (int i = 0; i < p->n_native; ++i)
+    zn[i] = fct_tables::slice_z(p->recon.start_pos, p->recon.end_pos, p->csw, i);
+  return zn;
+}
+
+}  // namespace
+
+namespace fct {
+
+int plan_set_tile_z(fct_wfbp_gpu_plan* p, int tz) {
+  if (tz != 4 && tz != 8 && tz != 16 && tz != 32) return FCT_ERR_INVALID_ARGUMENT;
+  if (tz == p->tile_z) return FCT_OK;
+  std::vector<int2> vr;
+  if (!slab_view_ranges(p, native_slice_z(p), tz, &vr))
+    return set_err(p->ctx, "a slab's view range exceeds 4 GiB of filtered rows"), FCT_ERR_LIMIT;
+  int2* d = nullptr;
+  HIP_TRY(p->ctx, upload(&d, vr, p->stream));
+  if (p->d_slab_vrange) (void)hipFree(p->d_slab_vrange);
+  p->d_slab_vrange = d;
+  p->tile_z = tz;
+  p->nslabs = (int)vr.size();
+  return FCT_OK;
+}
+
+BpArgs plan_bp_args(const fct_wfbp_gpu_plan* p) {
+  BpArgs a{};
   a.xs = p->d_xs;
   a.ys = p->d_ys;
   a.cs = p->d_cs;
@@ -164,6 +214,12 @@ fct::BpArgs bp_args(fct_wfbp_gpu_plan* p) {
   a.nslabs = p->nslabs;
   return a;
 }
+
+}  // namespace fct
+
+namespace {
+
+fct::BpArgs bp_args(const fct_wfbp_gpu_plan* p) { return fct::plan_bp_args(p); }
 
 }  // namespace
 
@@ -288,22 +344,10 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   std::vector<float2> cs(p->H);
   for (int t = 0; t < p->H; ++t) fct_tables::view_cos_sin(s.angle0, p->dtheta, t, &cs[t].x, &cs[t].y);
 
-  // Views a slab can select: table z within zrot of a slice (plus margin).
-  std::vector<int2> vr(p->nslabs);
-  for (int sl = 0; sl < p->nslabs; ++sl) {
-    const int a0 = sl * p->tile_z, a1 = std::min(p->n_native, a0 + p->tile_z) - 1;
-    const double zlo = std::min(zn[a0], zn[a1]) - std::fabs(s.feed);
-    const double zhi = std::max(zn[a0], zn[a1]) + std::fabs(s.feed);
-    double va = (zlo - s.z0) / p->dz, vb = (zhi - s.z0) / p->dz;
-    if (va > vb) std::swap(va, vb);
-    const double lo = std::floor(va) - 2, hi = std::ceil(vb) + 3;
-    vr[sl].x = (int)std::max(0.0, std::min(lo, (double)p->nviews));
-    vr[sl].y = (int)std::max(0.0, std::min(hi, (double)p->nviews));
-    // One buffer resource addresses a slab's views (32-bit byte offsets).
-    if ((size_t)(vr[sl].y - vr[sl].x) * p->nrows * p->cpitch * 4 >= ((size_t)1 << 32))
-      return fail((set_err(ctx, "a slab's view range exceeds 4 GiB of filtered rows"),
-                   FCT_ERR_LIMIT));
-  }
+  std::vector<int2> vr;
+  if (!slab_view_ranges(p, zn, p->tile_z, &vr))
+    return fail((set_err(ctx, "a slab's view range exceeds 4 GiB of filtered rows"),
+                 FCT_ERR_LIMIT));
 
   // Exact view-window slice ranges per (chunk, view): the slices s of the
   // chunk with fct_bp_in_window(zt, zs) true form one interval (the test is

@@ -55,6 +55,15 @@ struct BpArgs {
   int32_t sel_ix, sel_iy, sel_is;
 };
 
+// Persistent multi-job backprojection (backproject_v1.hip).
+struct BpBatchArgs {
+  const BpArgs* jobs;        // [n_jobs] device copy of each job's arguments
+  const int32_t* tile_start; // [n_jobs + 1] first global tile of each job
+  uint32_t* queue;           // [8] per-range tile counters, zero at launch
+  int32_t n_jobs;            // <= 64
+  int32_t total_tiles;
+};
+
 struct ThickenArgs {
   const float* native;
   float* out;
@@ -96,11 +105,22 @@ hipError_t launch_backproject(const BpArgs& a, int tile_z, int variant, int prec
                               hipStream_t s);
 hipError_t launch_backproject_v1(const BpArgs& a, int tile_z, int precision, int mode,
                                  hipStream_t s);
+hipError_t launch_backproject_batch(const BpBatchArgs& b, int tile_z, int precision,
+                                    bool centered, int grid, hipStream_t s);
+hipError_t backproject_batch_waves_per_cu(int tile_z, int precision, int* waves);
 hipError_t launch_thicken(const ThickenArgs& a, hipStream_t s);
 hipError_t launch_phantom(const PhantomArgs& a, hipStream_t s);
 
 enum BpMode { kBpAccumulate = 0, kBpCount = 1, kBpSelect = 2 };
 
+}  // namespace fct
+
+struct fct_wfbp_gpu_plan;
+
+namespace fct {
+// Plan helpers shared by capi.hip and batch.hip.
+BpArgs plan_bp_args(const fct_wfbp_gpu_plan* p);   // tables only (no buffers)
+int plan_set_tile_z(fct_wfbp_gpu_plan* p, int tz);  // rebuilds the slab view ranges
 }  // namespace fct
 
 struct fct_wfbp_gpu_ctx {

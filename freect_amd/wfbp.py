@@ -58,6 +58,11 @@ class PlanInfoC(C.Structure):
                 ("z_rot", C.c_double), ("tile_z", C.c_int32), ("bp_variant", C.c_int32)]
 
 
+class BatchInfoC(C.Structure):
+    _fields_ = [("n_jobs", C.c_int32), ("tile_z", C.c_int32), ("total_tiles", C.c_int32),
+                ("grid", C.c_int32), ("device_bytes", C.c_int64)]
+
+
 class EllipsoidC(C.Structure):
     _fields_ = [("c", C.c_double * 3), ("a", C.c_double * 3), ("rot", C.c_double * 9),
                 ("dmu", C.c_double)]
@@ -89,6 +94,16 @@ SIGNATURES = {
     "fct_wfbp_gpu_phantom_project": (C.c_int, [_VP, _P(ScanGeometryC), C.c_double, C.c_double,
                                                _P(EllipsoidC), C.c_int32, C.c_double,
                                                C.c_uint64, C.c_int32, C.c_int32, _VP, _VP]),
+    "fct_wfbp_gpu_batch_create": (C.c_int, [_VP, C.c_int32, _P(ScanGeometryC),
+                                            _P(ReconConfigC), _P(ExecOptionsC), _VP, _P(_VP)]),
+    "fct_wfbp_gpu_batch_destroy": (C.c_int, [_VP]),
+    "fct_wfbp_gpu_batch_submit": (C.c_int, [_VP, C.c_int32, _VP, _VP]),
+    "fct_wfbp_gpu_batch_run": (C.c_int, [_VP, C.c_int32]),
+    "fct_wfbp_gpu_batch_synchronize": (C.c_int, [_VP]),
+    "fct_wfbp_gpu_batch_autotune": (C.c_int, [_VP, _P(C.c_int32), C.c_int32, C.c_int32,
+                                              _P(C.c_float)]),
+    "fct_wfbp_gpu_batch_info": (C.c_int, [_VP, _P(BatchInfoC)]),
+    "fct_wfbp_gpu_batch_job_plan": (C.c_int, [_VP, C.c_int32, _P(_VP)]),
 }
 
 _lib = None
@@ -207,10 +222,22 @@ class Plan:
         ctx.check(load_library().fct_wfbp_gpu_plan_info(self.h, C.byref(info)), "plan_info")
         self.info = info
 
+    @classmethod
+    def _borrowed(cls, ctx: Context, h, scan, recon, precision):
+        """A view of a plan owned by a Batch (never destroyed from here)."""
+        self = cls.__new__(cls)
+        self.ctx, self.scan, self.recon, self.precision = ctx, scan, recon, precision
+        self.h = h
+        self._owner = False
+        info = PlanInfoC()
+        ctx.check(load_library().fct_wfbp_gpu_plan_info(self.h, C.byref(info)), "plan_info")
+        self.info = info
+        return self
+
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owner", True):
             load_library().fct_wfbp_gpu_plan_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -335,3 +362,70 @@ class Plan:
 
     def synchronize(self):
         self.ctx.check(load_library().fct_wfbp_gpu_synchronize(self.h), "synchronize")
+
+
+class Batch:
+    """Uneven reconstructions run together: per-job rebin_filter, one
+    persistent backprojection over every job's tiles, per-job thicken,
+    recorded into a hipGraph on first run (fct_wfbp_gpu_batch_*)."""
+
+    GRAPH, EAGER = 0, 1
+
+    def __init__(self, ctx: Context, jobs, precision: str = "fp32", tile_z: int = 0,
+                 stream=None):
+        """jobs: sequence of (ScanGeometry, ReconConfig)."""
+        self.ctx = ctx
+        self.precision = precision
+        self.jobs = list(jobs)
+        n = len(self.jobs)
+        scans = (ScanGeometryC * n)(*[scan_struct(s) for s, _ in self.jobs])
+        recons = (ReconConfigC * n)(*[recon_struct(r) for _, r in self.jobs])
+        opts = ExecOptionsC(PRECISION[precision], tile_z, 0, 0)
+        h = C.c_void_p()
+        ctx.check(load_library().fct_wfbp_gpu_batch_create(ctx.h, n, scans, recons,
+                                                           C.byref(opts),
+                                                           _stream_handle(stream),
+                                                           C.byref(h)), "batch_create")
+        self.h = h
+        self.plans = []
+        for j, (s, r) in enumerate(self.jobs):
+            ph = C.c_void_p()
+            ctx.check(load_library().fct_wfbp_gpu_batch_job_plan(self.h, j, C.byref(ph)),
+                      "batch_job_plan")
+            self.plans.append(Plan._borrowed(ctx, ph, s, r, precision))
+
+    def close(self):
+        if getattr(self, "h", None):
+            for p in self.plans:
+                p.h = None
+            load_library().fct_wfbp_gpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        i = BatchInfoC()
+        self.ctx.check(load_library().fct_wfbp_gpu_batch_info(self.h, C.byref(i)), "batch_info")
+        return {k: getattr(i, k) for k, _ in BatchInfoC._fields_}
+
+    def submit(self, job: int, raw, volume):
+        self.ctx.check(load_library().fct_wfbp_gpu_batch_submit(self.h, job, _ptr(raw),
+                                                                _ptr(volume)), "batch_submit")
+
+    def run(self, mode: int = 0):
+        self.ctx.check(load_library().fct_wfbp_gpu_batch_run(self.h, mode), "batch_run")
+
+    def synchronize(self):
+        self.ctx.check(load_library().fct_wfbp_gpu_batch_synchronize(self.h), "batch_sync")
+
+    def autotune(self, candidates=(4, 8, 16), reps: int = 2) -> dict:
+        n = len(candidates)
+        tz = (C.c_int32 * n)(*candidates)
+        ms = (C.c_float * n)()
+        self.ctx.check(load_library().fct_wfbp_gpu_batch_autotune(self.h, tz, n, reps, ms),
+                       "batch_autotune")
+        return {int(t): float(m) for t, m in zip(candidates, ms)}

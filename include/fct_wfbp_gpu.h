@@ -205,6 +205,58 @@ FCT_API int fct_wfbp_gpu_phantom_project(fct_wfbp_gpu_ctx* ctx, const fct_scan_g
                                          double i0, uint64_t seed, int32_t v0, int32_t v1,
                                          float* raw, void* stream);
 
+/* ---- Batched reconstruction of uneven jobs --------------------------------
+ * No counterpart in the reference: main.cpp:59-66 runs one
+ * ReconstructionMachine (ReconstructionMachine.h:16-21) per process, one
+ * volume at a time.  A batch owns one plan per job (same device, same
+ * precision and tile_z, any geometry / matrix / slice range / filter) plus
+ * every intermediate buffer.  One run is: rebin_filter of every job, ONE
+ * persistent backprojection launch over the tiles of all jobs (work queue per
+ * XCD), thicken of every job -- recorded once into a hipGraph and replayed.
+ * Each job's volume is bitwise identical to fct_wfbp_gpu_reconstruct on a
+ * plan with the same options. */
+typedef struct fct_wfbp_gpu_batch fct_wfbp_gpu_batch;
+
+typedef struct {
+  int32_t n_jobs;
+  int32_t tile_z;          /* current slices per wave tile (set by autotune) */
+  int32_t total_tiles;     /* backprojection tiles over all jobs */
+  int32_t grid;            /* persistent workgroups of the backprojection */
+  int64_t device_bytes;    /* filtered + scale + native buffers held by the batch */
+} fct_batch_info;
+
+typedef enum {
+  FCT_BATCH_RUN_GRAPH = 0,   /* replay the recorded hipGraph (record on first use) */
+  FCT_BATCH_RUN_EAGER = 1    /* launch the same kernels directly */
+} fct_batch_run_mode;
+
+/* opts: precision and tile_z for every job (bp_variant is ignored: the batch
+ * uses the lean kernel).  stream: NULL creates a stream owned by the batch
+ * (hipGraph capture needs one).  At most 64 jobs. */
+FCT_API int fct_wfbp_gpu_batch_create(fct_wfbp_gpu_ctx* ctx, int32_t n_jobs,
+                                      const fct_scan_geometry* scans,
+                                      const fct_recon_config* recons,
+                                      const fct_exec_options* opts, void* stream,
+                                      fct_wfbp_gpu_batch** batch);
+FCT_API int fct_wfbp_gpu_batch_destroy(fct_wfbp_gpu_batch* batch);
+/* Bind job's device input raw [view][row][ch] and output volume
+ * [slice_req][y][x].  Rebinding invalidates the recorded graph. */
+FCT_API int fct_wfbp_gpu_batch_submit(fct_wfbp_gpu_batch* batch, int32_t job, const float* raw,
+                                      float* volume);
+/* Enqueue one run on the batch stream (asynchronous). */
+FCT_API int fct_wfbp_gpu_batch_run(fct_wfbp_gpu_batch* batch, int32_t mode);
+FCT_API int fct_wfbp_gpu_batch_synchronize(fct_wfbp_gpu_batch* batch);
+/* Autotuner: runs rebin_filter of every job once, then times the persistent
+ * backprojection `reps` times for each candidate tile_z (HIP events on the
+ * batch stream), keeps the fastest and returns each candidate's mean ms in
+ * ms_out (host, n entries, may be NULL).  Synchronous; needs every job bound. */
+FCT_API int fct_wfbp_gpu_batch_autotune(fct_wfbp_gpu_batch* batch, const int32_t* tile_z,
+                                        int32_t n, int32_t reps, float* ms_out);
+FCT_API int fct_wfbp_gpu_batch_info(const fct_wfbp_gpu_batch* batch, fct_batch_info* info);
+/* The plan of one job (owned by the batch), for plan_info / counting. */
+FCT_API int fct_wfbp_gpu_batch_job_plan(fct_wfbp_gpu_batch* batch, int32_t job,
+                                        fct_wfbp_gpu_plan** plan);
+
 #ifdef __cplusplus
 }
 #endif
