@@ -114,20 +114,24 @@ __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
   }
 }
 
-// Transpose of the rebinning: one block per parallel line (j, row), threads
-// over its channels.
+// Transpose of the rebinning: one block per parallel line (j, effective row
+// q), threads over its channels; the same clamped (focal state, projection,
+// raw row, fan channel) addressing as rebin_filter.hip's gather.
 __global__ __launch_bounds__(256) void rebin_adjoint_kernel(RebinAdjArgs a) {
-  const int line = blockIdx.x;             // j * nrows + row
-  const int j = line / a.nrows, row = line - j * a.nrows;
+  const int line = blockIdx.x;             // j * nrows + q
+  const int j = line / a.nrows, q = line - j * a.nrows;
+  const int row = q >> a.zshift;
+  const RebinDev* tab = a.tab + (size_t)(q & a.zshift) * a.ncols;
   for (int c = threadIdx.x; c < a.ncols; c += 256) {
-    const RebinDev e = a.tab[c];
-    if (!e.valid) continue;
+    const RebinDev e = tab[c];
+    if (!e.state) continue;
     const float y = a.gbar[(size_t)line * a.cpitch + c];
     if (y == 0.0f) continue;
-    const int va = min(max(j + e.aoff, 0), a.nviews - 1);
-    const int vb = min(max(j + e.aoff + 1, 0), a.nviews - 1);
-    float* pa = a.pbar + ((size_t)va * a.nrows + row) * a.ncols;
-    float* pb = a.pbar + ((size_t)vb * a.nrows + row) * a.ncols;
+    const int rho = e.state - 1;
+    const int va = min(max(j + e.aoff, 0), e.mmax) * a.states + rho;
+    const int vb = min(max(j + e.aoff + 1, 0), e.mmax) * a.states + rho;
+    float* pa = a.pbar + ((size_t)va * a.raw_nrows + row) * a.raw_ncols;
+    float* pb = a.pbar + ((size_t)vb * a.raw_nrows + row) * a.raw_ncols;
     unsafeAtomicAdd(pa + e.b0, (1.0f - e.fb) * (1.0f - e.fa) * y);
     unsafeAtomicAdd(pb + e.b0, (1.0f - e.fb) * e.fa * y);
     unsafeAtomicAdd(pa + e.b1, e.fb * (1.0f - e.fa) * y);

@@ -77,14 +77,34 @@ __global__ __launch_bounds__(256) void phantom_kernel(PhantomArgs a) {
   const int v = a.v0 + blockIdx.z;
   if (c >= a.ncols) return;
   const double ang = a.angle0 + a.dtheta * (double)v;
-  const double sx = a.r * cos(ang), sy = a.r * sin(ang);
-  const double sz = a.z0 + a.dz * (double)v;
+  double sx = a.r * cos(ang), sy = a.r * sin(ang);
+  double sz = a.z0 + a.dz * (double)v;
   const double w = ((double)row - a.cr) * a.drow;
   const double beta = ((double)c - a.cc) * a.dgamma;
   const double dx = -a.dsd * cos(ang + beta);
   const double dy = -a.dsd * sin(ang + beta);
-  const double len = sqrt(a.dsd * a.dsd + w * w);
-  const double ux = dx / len, uy = dy / len, uz = w / len;
+  double ux, uy, uz;
+  if (a.states == 1) {
+    const double len = sqrt(a.dsd * a.dsd + w * w);
+    ux = dx / len;
+    uy = dy / len;
+    uz = w / len;
+  } else {
+    // Flying focal spot: the focal spot of state v mod S moves by
+    // (dr, da, dz) along (e_w, e_u, e_z); the detector cell stays where the
+    // undeflected source puts it (generate_system_matrix_ffs.cpp:126-149).
+    const double* o = a.ffs_off[v % a.states];
+    const double ca = cos(ang), sa = sin(ang);
+    const double ox = o[0] * ca - o[1] * sa, oy = o[0] * sa + o[1] * ca;
+    sx += ox;
+    sy += oy;
+    sz += o[2];
+    const double ex = dx - ox, ey = dy - oy, ez = w - o[2];
+    const double len = sqrt(ex * ex + ey * ey + ez * ez);
+    ux = ex / len;
+    uy = ey / len;
+    uz = ez / len;
+  }
   double acc = 0.0;
   {
     const double A = ux * ux + uy * uy;

@@ -115,6 +115,19 @@ int validate(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* s, const fct_recon_
     return set_err(ctx, "invalid distances or spacings"), FCT_ERR_INVALID_ARGUMENT;
   if (!(std::fabs(s->feed) > 0))
     return set_err(ctx, "table feed must be nonzero (helical scans only)"), FCT_ERR_UNSUPPORTED;
+  if (s->ffs_mode < FCT_FFS_NONE || s->ffs_mode > FCT_FFS_PHI_Z)
+    return set_err(ctx, "unknown ffs_mode %d", s->ffs_mode), FCT_ERR_INVALID_ARGUMENT;
+  if (s->ffs_mode != FCT_FFS_NONE) {
+    const int S = fct_ffs::n_states(s->ffs_mode);
+    if (s->views_per_rotation % (2 * S))
+      return set_err(ctx, "with FFS, views_per_rotation must be a multiple of %d", 2 * S),
+             FCT_ERR_UNSUPPORTED;
+    if (s->n_views < S) return set_err(ctx, "fewer views than focal states"), FCT_ERR_INVALID_ARGUMENT;
+    if (!(s->anode_angle > 0.0 && s->anode_angle < 1.5))
+      return set_err(ctx, "anode_angle must be in (0, 1.5) rad"), FCT_ERR_INVALID_ARGUMENT;
+    if (fct_ffs::has_phi(s->ffs_mode) && 2 * s->n_channels > 2048)
+      return set_err(ctx, "interleaved channels %d exceed 2048", 2 * s->n_channels), FCT_ERR_LIMIT;
+  }
   if (r->nx < 1 || r->ny < 1 || r->nx > 8192 || r->ny > 8192)
     return set_err(ctx, "nx, ny must be in [1, 8192]"), FCT_ERR_INVALID_ARGUMENT;
   if (!(r->recon_fov > 0) || !(r->slice_thickness > 0))
@@ -271,7 +284,22 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   if (!p) return FCT_ERR_OUT_OF_MEMORY;
   p->ctx = ctx;
   p->device = ctx->device;
+  p->raw_scan = *scan;
+  p->ffs = fct_ffs::make_geometry(scan->ffs_mode, scan->n_channels, scan->n_rows, scan->n_views,
+                                  scan->views_per_rotation, scan->channel_spacing,
+                                  scan->row_spacing, scan->central_channel, scan->central_row,
+                                  scan->r, scan->dsd, scan->anode_angle);
+  // Everything downstream of the rebinning sees the interleaved detector.
   p->scan = *scan;
+  p->scan.n_channels = p->ffs.e_ncols;
+  p->scan.n_rows = p->ffs.e_nrows;
+  p->scan.n_views = p->ffs.e_nviews;
+  p->scan.views_per_rotation = p->ffs.e_ppr;
+  p->scan.channel_spacing = p->ffs.e_dch;
+  p->scan.row_spacing = p->ffs.e_drow;
+  p->scan.central_channel = p->ffs.e_cc;
+  p->scan.central_row = p->ffs.e_cr;
+  p->scan.ffs_mode = FCT_FFS_NONE;
   p->recon = *recon;
   if (opts) p->opts = *opts;
   p->stream = static_cast<hipStream_t>(stream);   // NULL: the null stream
@@ -292,7 +320,7 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
     }                                                                              \
   } while (0)
 
-  const fct_scan_geometry& s = *scan;
+  const fct_scan_geometry& s = p->scan;
   const fct_recon_config& r = *recon;
   p->ncols = s.n_channels;
   p->nrows = s.n_rows;
@@ -368,13 +396,17 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
     }
   }
 
-  // Rebinning table (cuda_kernels.cuh:17-36).
-  std::vector<fct::RebinDev> tab(p->ncols);
-  for (int c = 0; c < p->ncols; ++c) {
-    const fct_tables::RebinEntry e =
-        fct_tables::rebin_entry(c, s.central_channel, p->dgamma, p->dtheta, p->ncols);
-    tab[c] = fct::RebinDev{e.b0, e.b1, e.aoff, e.valid, (float)e.fb, (float)e.fa};
-  }
+  // Rebinning table (cuda_kernels.cuh:17-36), one per effective-row parity
+  // with z FFS; without FFS exactly fct_tables::rebin_entry.
+  const int zpar = fct_ffs::has_z(p->ffs.mode) ? 2 : 1;
+  std::vector<fct::RebinDev> tab((size_t)zpar * p->ncols);
+  for (int zb = 0; zb < zpar; ++zb)
+    for (int c = 0; c < p->ncols; ++c) {
+      const fct_ffs::Entry e = fct_ffs::rebin_entry(p->ffs, c, zb);
+      tab[(size_t)zb * p->ncols + c] =
+          fct::RebinDev{e.b0, e.b1, e.aoff, e.valid ? e.rho + 1 : 0, (float)e.fb, (float)e.fa,
+                        e.mmax, 0};
+    }
 
   // Filter spectrum (generate_filter, ReconstructionMachine.cu:478-544):
   // taps -> fftshift -> DFT in FP64; the spectrum of the symmetric kernel is
@@ -475,7 +507,7 @@ FCT_API int fct_wfbp_gpu_plan_info(const fct_wfbp_gpu_plan* p, fct_plan_info* in
   info->n_slices_requested = p->n_req;
   info->nfft = p->nfft;
   info->channel_pitch = p->cpitch;
-  info->raw_elems = (int64_t)p->nviews * p->nrows * p->ncols;
+  info->raw_elems = p->raw_elems();
   info->filtered_bytes = (int64_t)p->filtered_bytes();
   info->row_scale_elems = (int64_t)p->row_scale_elems();
   info->native_elems = (int64_t)p->n_native * p->recon.ny * p->recon.nx;
@@ -485,6 +517,10 @@ FCT_API int fct_wfbp_gpu_plan_info(const fct_wfbp_gpu_plan* p, fct_plan_info* in
   info->z_rot = std::fabs(p->scan.feed);
   info->tile_z = p->tile_z;
   info->bp_variant = p->variant;
+  info->n_parallel_views = p->nviews;
+  info->n_filtered_rows = p->nrows;
+  info->n_filtered_channels = p->ncols;
+  info->ffs_states = p->ffs.S;
   return FCT_OK;
 }
 
@@ -516,6 +552,10 @@ FCT_API int fct_wfbp_gpu_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, vo
   a.precision = p->opts.precision;
   a.nlines = p->nviews * p->nrows;
   a.ncu = p->ncu;
+  a.raw_nrows = p->raw_scan.n_rows;
+  a.raw_ncols = p->raw_scan.n_channels;
+  a.states = p->ffs.S;
+  a.zshift = fct_ffs::has_z(p->ffs.mode) ? 1 : 0;
   HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
   return FCT_OK;
 }
@@ -568,6 +608,10 @@ FCT_API int fct_wfbp_gpu_filter(fct_wfbp_gpu_plan* p, const float* lines_in, flo
   a.precision = FCT_PRECISION_FP32;
   a.nlines = p->nviews * p->nrows;
   a.ncu = p->ncu;
+  a.raw_nrows = p->raw_scan.n_rows;
+  a.raw_ncols = p->raw_scan.n_channels;
+  a.states = p->ffs.S;
+  a.zshift = fct_ffs::has_z(p->ffs.mode) ? 1 : 0;
   HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
   return FCT_OK;
 }
@@ -590,8 +634,7 @@ FCT_API int fct_wfbp_gpu_rebin_adjoint(fct_wfbp_gpu_plan* p, const float* filter
                                        float* raw_bar) {
   if (!p || !filtered_bar || !raw_bar) return FCT_ERR_INVALID_ARGUMENT;
   HIP_TRY(p->ctx, hipSetDevice(p->device));
-  HIP_TRY(p->ctx, fct::launch_fill(raw_bar, (size_t)p->nviews * p->nrows * p->ncols, 0.0f,
-                                   p->stream));
+  HIP_TRY(p->ctx, fct::launch_fill(raw_bar, (size_t)p->raw_elems(), 0.0f, p->stream));
   fct::RebinAdjArgs a{};
   a.gbar = filtered_bar;
   a.pbar = raw_bar;
@@ -600,6 +643,10 @@ FCT_API int fct_wfbp_gpu_rebin_adjoint(fct_wfbp_gpu_plan* p, const float* filter
   a.nrows = p->nrows;
   a.ncols = p->ncols;
   a.cpitch = p->cpitch;
+  a.raw_nrows = p->raw_scan.n_rows;
+  a.raw_ncols = p->raw_scan.n_channels;
+  a.states = p->ffs.S;
+  a.zshift = fct_ffs::has_z(p->ffs.mode) ? 1 : 0;
   HIP_TRY(p->ctx, fct::launch_rebin_adjoint(a, p->stream));
   return FCT_OK;
 }
@@ -636,7 +683,7 @@ FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* p, const float* raw, flo
                                      int32_t host_buffers) {
   if (!p || !raw || !volume) return FCT_ERR_INVALID_ARGUMENT;
   HIP_TRY(p->ctx, hipSetDevice(p->device));
-  const size_t raw_bytes = (size_t)p->nviews * p->nrows * p->ncols * sizeof(float);
+  const size_t raw_bytes = (size_t)p->raw_elems() * sizeof(float);
   const size_t nat_bytes = (size_t)p->n_native * p->recon.ny * p->recon.nx * sizeof(float);
   const size_t vol_bytes = (size_t)p->n_req * p->recon.ny * p->recon.nx * sizeof(float);
   if (!p->d_filtered) HIP_TRY(p->ctx, hipMalloc(&p->d_filtered, p->filtered_bytes()));
@@ -738,6 +785,9 @@ FCT_API int fct_wfbp_gpu_phantom_project(fct_wfbp_gpu_ctx* ctx, const fct_scan_g
   if (scan->views_per_rotation < 1 || scan->n_channels < 1 || scan->n_rows < 1 ||
       scan->n_rows > 65535)
     return set_err(ctx, "bad scan geometry"), FCT_ERR_INVALID_ARGUMENT;
+  if (scan->ffs_mode < FCT_FFS_NONE || scan->ffs_mode > FCT_FFS_PHI_Z ||
+      (scan->ffs_mode != FCT_FFS_NONE && !(scan->anode_angle > 0.0 && scan->anode_angle < 1.5)))
+    return set_err(ctx, "bad ffs_mode / anode_angle"), FCT_ERR_INVALID_ARGUMENT;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   hipStream_t s = static_cast<hipStream_t>(stream);
   fct_ellipsoid* d_ell = nullptr;
@@ -773,6 +823,17 @@ FCT_API int fct_wfbp_gpu_phantom_project(fct_wfbp_gpu_ctx* ctx, const fct_scan_g
   a.cyl_mu = cyl_mu;
   a.i0 = i0;
   a.seed = seed;
+  const fct_ffs::Geometry fg = fct_ffs::make_geometry(
+      scan->ffs_mode, scan->n_channels, scan->n_rows, scan->n_views, scan->views_per_rotation,
+      scan->channel_spacing, scan->row_spacing, scan->central_channel, scan->central_row,
+      scan->r, scan->dsd, scan->anode_angle);
+  a.states = fg.S;
+  for (int rho = 0; rho < fg.S; ++rho) {
+    const fct_ffs::Deflection d = fct_ffs::state_offset(fg.mode, rho, fg.dr, fg.da, fg.anode);
+    a.ffs_off[rho][0] = d.dr;
+    a.ffs_off[rho][1] = d.da;
+    a.ffs_off[rho][2] = d.dz;
+  }
   hipError_t e = fct::launch_phantom(a, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (d_ell) (void)hipFree(d_ell);

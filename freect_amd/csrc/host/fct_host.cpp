@@ -382,13 +382,18 @@ void fct_make_phantom(uint64_t seed, double z_lo, double z_hi, int n,
 bool fct_synthetic_workload(const std::string& name, uint64_t seed, fct_scan_geometry* g,
                             fct_recon_config* rc, double* i0, std::vector<fct_ellipsoid>* ell,
                             uint64_t* seed_used) {
-  const double R = 541.0, Dsd = 949.0;
-  int nch, nrows, ppr;
+  double R = 541.0, Dsd = 949.0, drow_iso = 0.625, cc_off = 0.25, pitch_z = 0.0;
+  int nch, nrows, ppr, ffs = 0;
   double dgamma, pitch, rotations, zlo, zhi, fov, thick;
   int nx;
   std::string filt;
   uint64_t s;
-  if (name == "small") {
+  if (name == "ffs") {   // scans.ffs(): Siemens-like, phi + z FFS (definitionas.scanner:5-14)
+    R = 595.0; Dsd = 1085.6; drow_iso = 0.6; cc_off = 366.25 - 367.5;
+    nch = 736; nrows = 32; ppr = 2304; dgamma = 0.0011844; ffs = FCT_FFS_PHI_Z;
+    pitch = 0.8; rotations = 8; zlo = -50; zhi = 50; fov = 500; thick = 1.0; nx = 512;
+    pitch_z = 0.5; filt = "sharp"; s = 1002; *i0 = 2.5e4;
+  } else if (name == "small") {
     nch = 736; nrows = 16; ppr = 1152; dgamma = 2.0 * std::asin(250.0 / R) / 735.0;
     pitch = 1.0; rotations = 2; zlo = -20; zhi = 20; fov = 250; thick = 1.25; nx = 256;
     filt = "shepp-logan"; s = 1000; *i0 = 0.0;
@@ -411,11 +416,13 @@ bool fct_synthetic_workload(const std::string& name, uint64_t seed, fct_scan_geo
   g->views_per_rotation = ppr;
   g->n_views = (int)std::lround(rotations * ppr);
   g->channel_spacing = dgamma * Dsd;
-  g->row_spacing = 0.625 * Dsd / R;
+  g->row_spacing = drow_iso * Dsd / R;
   g->dsd = Dsd;
   g->r = R;
-  g->central_channel = 0.5 * (nch - 1) + 0.25;
+  g->central_channel = 0.5 * (nch - 1) + cc_off;
   g->central_row = 0.5 * (nrows - 1);
+  g->ffs_mode = ffs;
+  g->anode_angle = ffs ? 0.1222 : 0.0;
   const double csw = R * g->row_spacing / Dsd;
   g->feed = pitch * nrows * csw;
   g->z0 = -0.5 * (g->n_views - 1) * g->feed / ppr;
@@ -425,7 +432,7 @@ bool fct_synthetic_workload(const std::string& name, uint64_t seed, fct_scan_geo
   rc->end_pos = zhi;
   rc->recon_fov = fov;
   rc->slice_thickness = thick;
-  rc->slice_pitch = thick;
+  rc->slice_pitch = pitch_z > 0.0 ? pitch_z : thick;
   rc->nx = rc->ny = nx;
   FilterSpec f{};
   filter_spec(filt, &f);
@@ -572,6 +579,15 @@ int ReconstructionMachine::ConfigureCTGeometry() {
     g.central_row = ds.getDetectorCentralRow();
     g.angle0 = ds.getTubeAngle(0);
     g.z0 = ds.getTablePosition(0);
+    // FlyingFocalSpotMode (DicomDataSet.cpp:193, 285): the DICOM-CT-PD values
+    // FFSNONE / FFSXY / FFSZ / FFSXYZ.  The anode angle is not part of
+    // RawDataSet; 7 degrees, the Siemens value, is assumed.
+    const std::string fm = ds.getFlyingFocalSpotMode();
+    g.ffs_mode = fm == "FFSXYZ" ? FCT_FFS_PHI_Z
+                 : fm == "FFSXY" ? FCT_FFS_PHI
+                 : fm == "FFSZ"  ? FCT_FFS_Z
+                                 : FCT_FFS_NONE;
+    g.anode_angle = g.ffs_mode ? 7.0 * 3.14159265358979323846 / 180.0 : 0.0;
     if (g.n_views > g.views_per_rotation)
       g.feed = (double)ds.getTablePosition(g.views_per_rotation) - g.z0;
     else if (g.n_views > 1)

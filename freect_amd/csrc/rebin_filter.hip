@@ -28,22 +28,29 @@ namespace {
 
 constexpr int kRfThreads = 256;
 
-__device__ __forceinline__ float rebin_sample(const float* __restrict__ raw, int j, int row,
-                                              const RebinDev& e, int nviews, int nrows,
-                                              int ncols) {
-  // Bilinear at (fan view j + aoff + fa, fan channel b0 + fb) with the view
-  // index clamped to the scan, as the reference's clamp-addressed texture.
-  int va = j + e.aoff;
-  int vb = va + 1;
-  va = min(max(va, 0), nviews - 1);
-  vb = min(max(vb, 0), nviews - 1);
-  const float* ra = raw + ((size_t)va * nrows + row) * ncols;
-  const float* rb = raw + ((size_t)vb * nrows + row) * ncols;
+__device__ __forceinline__ float rebin_sample(const float* __restrict__ raw, int j, int q,
+                                              const RebinDev& e, const RebinFilterArgs& a) {
+  // Bilinear at (projection j + aoff + fa of the entry's focal state, fan
+  // channel b0 + fb) with the projection clamped to the scan, as the
+  // reference's clamp-addressed texture; without FFS the state is 0 of 1 and
+  // the projection is the fan view.
+  const int rho = e.state - 1;
+  const int ma = min(max(j + e.aoff, 0), e.mmax);
+  const int mb = min(max(j + e.aoff + 1, 0), e.mmax);
+  const int row = q >> a.zshift;
+  const float* ra = raw + ((size_t)(ma * a.states + rho) * a.raw_nrows + row) * a.raw_ncols;
+  const float* rb = raw + ((size_t)(mb * a.states + rho) * a.raw_nrows + row) * a.raw_ncols;
   const float a0 = ra[e.b0], a1 = ra[e.b1];
   const float c0 = rb[e.b0], c1 = rb[e.b1];
   const float lo = fmaf(e.fa, c0 - a0, a0);
   const float hi = fmaf(e.fa, c1 - a1, a1);
   return fmaf(e.fb, hi - lo, lo);
+}
+
+// Table entry of channel c for effective row q (one table per row parity
+// with z FFS).
+__device__ __forceinline__ const RebinDev& rebin_entry_of(const RebinFilterArgs& a, int q, int c) {
+  return a.tab[(size_t)(q & a.zshift) * a.ncols + c];
 }
 
 __device__ FctCplx* fft_forward(FctCplx* a, FctCplx* b, const FctCplx* __restrict__ tw, int N,
@@ -92,10 +99,12 @@ __global__ __launch_bounds__(kRfThreads) void rebin_filter_kernel(RebinFilterArg
         if (has1) vb = a.lines_in[(size_t)l1 * a.cpitch + c];
       }
     } else if (c < a.ncols) {
-      const RebinDev e = a.tab[c];
-      if (e.valid) {   // fan channel outside [0, ncols): 0 (cuda_kernels.cuh:32-34)
-        va = rebin_sample(a.raw, j0, r0, e, a.nviews, a.nrows, a.ncols);
-        if (has1) vb = rebin_sample(a.raw, j1, r1, e, a.nviews, a.nrows, a.ncols);
+      // No fan sample (fan channel outside [0, ncols)): 0 (cuda_kernels.cuh:32-34).
+      const RebinDev e0 = rebin_entry_of(a, r0, c);
+      if (e0.state) va = rebin_sample(a.raw, j0, r0, e0, a);
+      if (has1) {
+        const RebinDev e1 = rebin_entry_of(a, r1, c);
+        if (e1.state) vb = rebin_sample(a.raw, j1, r1, e1, a);
       }
     }
     buf0[c] = FctCplx{va, vb};
@@ -253,10 +262,12 @@ __global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs
           if (has1) vb = a.lines_in[(size_t)l1 * a.cpitch + c];
         }
       } else if (c < a.ncols) {
-        const RebinDev e = a.tab[c];   // small table, cache resident
-        if (e.valid) {   // fan channel outside [0, ncols): 0 (cuda_kernels.cuh:32-34)
-          va = rebin_sample(a.raw, j0, r0, e, a.nviews, a.nrows, a.ncols);
-          if (has1) vb = rebin_sample(a.raw, j1, r1, e, a.nviews, a.nrows, a.ncols);
+        // Small tables, cache resident.  No fan sample: 0 (cuda_kernels.cuh:32-34).
+        const RebinDev e0 = rebin_entry_of(a, r0, c);
+        if (e0.state) va = rebin_sample(a.raw, j0, r0, e0, a);
+        if (has1) {
+          const RebinDev e1 = rebin_entry_of(a, r1, c);
+          if (e1.state) vb = rebin_sample(a.raw, j1, r1, e1, a);
         }
       }
       v[m] = FctCplx{va, vb};

@@ -7,15 +7,20 @@
 #include <vector>
 
 #include "../../include/fct_wfbp_gpu.h"
+#include "ffs_geometry.h"
 #include "wfbp_math.h"
 
 namespace fct {
 
-// Device-side rebinning entry per parallel channel (FP32 weights from the FP64
-// table recipe of wfbp_math.h).
+// Device-side rebinning entry per parallel channel and effective-row parity
+// (FP32 weights from the FP64 recipes of wfbp_math.h / ffs_geometry.h).
+// state: 0 = no fan sample (output 0), else 1 + the focal state rho; the
+// state's projection m = j + aoff (+1) is clamped to [0, mmax] and read at
+// raw view m * states + rho.
 struct RebinDev {
-  int32_t b0, b1, aoff, valid;
+  int32_t b0, b1, aoff, state;
   float fb, fa;
+  int32_t mmax, pad;
 };
 
 // Everything the rebin + filter kernel reads.
@@ -31,6 +36,9 @@ struct RebinFilterArgs {
   int32_t nlines;            // nviews * nrows
   const float* lines_in;     // filter-only mode: [line][cpitch] FP32 input (no rebinning)
   int32_t ncu;               // compute units of the device (persistent grid size)
+  int32_t raw_nrows, raw_ncols;   // raw detector (differs from nrows / ncols with FFS)
+  int32_t states;            // focal states S (1 without FFS)
+  int32_t zshift;            // 1 with z FFS: effective row q = 2 r + parity
 };
 
 // Everything the backprojection kernel reads.
@@ -82,6 +90,8 @@ struct PhantomArgs {
   double r, dsd, dgamma, drow, cc, cr, angle0, dtheta, z0, dz;
   double cyl_r, cyl_mu, i0;
   unsigned long long seed;
+  int32_t states;            // focal states (1: no FFS)
+  double ffs_off[4][3];      // per state (dr, da, dz) along (e_w, e_u, e_z)
 };
 
 struct BpAdjArgs {
@@ -91,10 +101,11 @@ struct BpAdjArgs {
 };
 
 struct RebinAdjArgs {
-  const float* gbar;         // [view][row][cpitch]
-  float* pbar;               // [view][row][ncols], accumulated into
+  const float* gbar;         // [view][row][cpitch] (parallel, effective detector)
+  float* pbar;               // [raw view][raw row][raw ch], accumulated into
   const RebinDev* tab;
-  int32_t nviews, nrows, ncols, cpitch;
+  int32_t nviews, nrows, ncols, cpitch;   // parallel views, effective rows / channels
+  int32_t raw_nrows, raw_ncols, states, zshift;
 };
 
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s);
@@ -132,7 +143,9 @@ struct fct_wfbp_gpu_plan {
   fct_wfbp_gpu_ctx* ctx = nullptr;
   int device = 0;
   hipStream_t stream = nullptr;
-  fct_scan_geometry scan{};
+  fct_scan_geometry scan{};         // interleaved (effective) geometry, ffs_mode none
+  fct_scan_geometry raw_scan{};     // as given (raw detector, FFS mode)
+  fct_ffs::Geometry ffs{};
   fct_recon_config recon{};
   fct_exec_options opts{};
 
@@ -169,6 +182,9 @@ struct fct_wfbp_gpu_plan {
   int32_t* d_thkT_idx = nullptr;
   float* d_thkT_w = nullptr;
 
+  int64_t raw_elems() const {
+    return (int64_t)raw_scan.n_views * raw_scan.n_rows * raw_scan.n_channels;
+  }
   size_t filtered_bytes() const {
     const size_t el = opts.precision == FCT_PRECISION_FP32 ? 4 : 2;
     return (size_t)nviews * nrows * cpitch * el;

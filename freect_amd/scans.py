@@ -64,6 +64,30 @@ class ScanGeometry:
     angle0: float = 0.0         # tube angle of view 0 (rad)
     z0: float = 0.0             # source z of view 0 (mm)
     feed: float = 0.0           # table feed per rotation (mm, signed)
+    ffs_mode: int = 0           # 0 none, 1 phi, 2 z, 3 phi+z (focal state = view mod S)
+    anode_angle: float = 0.0    # rad (z-FFS deflection)
+
+    @property
+    def ffs_states(self):
+        return {0: 1, 1: 2, 2: 2, 3: 4}[self.ffs_mode]
+
+    def effective(self) -> "ScanGeometry":
+        """The interleaved detector the reconstruction works on: phi FFS
+        doubles the channels (spacing / 2, central channel 2 cc + 1/2), z FFS
+        the rows (spacing / 2, central row 2 cr + 1/2), and S focal states
+        make n_views // S parallel views of S projection angles each."""
+        S = self.ffs_states
+        phi, z = bool(self.ffs_mode & 1), bool(self.ffs_mode & 2)
+        return dataclasses.replace(
+            self,
+            n_channels=self.n_channels * (2 if phi else 1),
+            channel_spacing=self.channel_spacing / (2 if phi else 1),
+            central_channel=2 * self.central_channel + 0.5 if phi else self.central_channel,
+            n_rows=self.n_rows * (2 if z else 1),
+            row_spacing=self.row_spacing / (2 if z else 1),
+            central_row=2 * self.central_row + 0.5 if z else self.central_row,
+            n_views=self.n_views // S, views_per_rotation=self.views_per_rotation // S,
+            ffs_mode=0)
 
     @property
     def dgamma(self):
@@ -272,6 +296,21 @@ def adjoint(seed: int = 1004) -> Workload:
     return Workload("adjoint", scan, rc, make_phantom(seed, -20.0, 20.0), 0.0, seed)
 
 
+def ffs(seed: int = 1002) -> Workload:
+    """Siemens-like scan with combined phi + z flying focal spot: 736 x 32
+    rows (0.6 mm at iso), 2304 projections per rotation in a 4-state focal
+    cycle (interleaved to 1472 channels x 64 rows, 576 parallel views per
+    rotation), pitch 0.8, 8 rotations, Poisson noise at I0 = 2.5e4; 512^2 x 200
+    slices of 1.0 mm at 0.5 mm spacing (the 100 mm the 123 mm of table travel
+    covers fully), sharp kernel (c = 1, a = 1)."""
+    det = siemens_detector()
+    det["views_per_rotation"] = 2304
+    scan = helical(det, pitch=0.8, rotations=8).replace(ffs_mode=3, anode_angle=0.1222)
+    rc = ReconConfig(start_pos=-50.0, end_pos=50.0, recon_fov=500.0, slice_thickness=1.0,
+                     slice_pitch=0.5, nx=512, ny=512, filter="sharp")
+    return Workload("ffs", scan, rc, make_phantom(seed, -50.0, 50.0), 2.5e4, seed)
+
+
 def batch_jobs(seed: int = 1003, n_jobs: int = 24) -> List[Workload]:
     """Uneven jobs on the 888x64 detector: scan lengths 2-20 rotations, pitch
     in {0.516, 0.984, 1.375}, matrix in {256^2, 512^2}, 16-240 slices of
@@ -303,7 +342,7 @@ def batch_jobs(seed: int = 1003, n_jobs: int = 24) -> List[Workload]:
 
 
 def tiny(seed: int = 7, n_channels=96, n_rows=8, ppr=96, rotations=2.5, nx=32,
-         n_slices=8, filter="ramp") -> Workload:
+         n_slices=8, filter="ramp", ffs_mode=0, anode_angle=0.1222) -> Workload:
     """A scaled-down GE-like scan for fast oracle tests."""
     R, Dsd = 541.0, 949.0
     dgamma = 2.0 * math.asin(250.0 / R) / (n_channels - 1)
@@ -312,7 +351,9 @@ def tiny(seed: int = 7, n_channels=96, n_rows=8, ppr=96, rotations=2.5, nx=32,
                central_channel=0.5 * (n_channels - 1) + 0.25,
                central_row=0.5 * (n_rows - 1))
     scan = helical(det, pitch=1.0, rotations=rotations, angle0=0.1)
-    csw = scan.csw
+    if ffs_mode:
+        scan = scan.replace(ffs_mode=ffs_mode, anode_angle=anode_angle)
+    csw = scan.effective().csw
     rc = ReconConfig(start_pos=-0.5 * n_slices * csw, end_pos=0.5 * n_slices * csw,
                      recon_fov=400.0, slice_thickness=csw, slice_pitch=csw, nx=nx, ny=nx,
                      filter=filter)

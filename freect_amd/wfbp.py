@@ -32,7 +32,9 @@ class ScanGeometryC(C.Structure):
                 ("views_per_rotation", C.c_int32), ("channel_spacing", C.c_double),
                 ("row_spacing", C.c_double), ("dsd", C.c_double), ("r", C.c_double),
                 ("central_channel", C.c_double), ("central_row", C.c_double),
-                ("angle0", C.c_double), ("z0", C.c_double), ("feed", C.c_double)]
+                ("angle0", C.c_double), ("z0", C.c_double), ("feed", C.c_double),
+                ("ffs_mode", C.c_int32), ("ffs_reserved", C.c_int32),
+                ("anode_angle", C.c_double)]
 
 
 class ReconConfigC(C.Structure):
@@ -55,7 +57,9 @@ class PlanInfoC(C.Structure):
                 ("filtered_bytes", C.c_int64), ("row_scale_elems", C.c_int64),
                 ("native_elems", C.c_int64), ("volume_elems", C.c_int64),
                 ("collimated_slice_width", C.c_double), ("acquisition_fov", C.c_double),
-                ("z_rot", C.c_double), ("tile_z", C.c_int32), ("bp_variant", C.c_int32)]
+                ("z_rot", C.c_double), ("tile_z", C.c_int32), ("bp_variant", C.c_int32),
+                ("n_parallel_views", C.c_int32), ("n_filtered_rows", C.c_int32),
+                ("n_filtered_channels", C.c_int32), ("ffs_states", C.c_int32)]
 
 
 class BatchInfoC(C.Structure):
@@ -128,7 +132,8 @@ def load_library():
 def scan_struct(scan: S.ScanGeometry) -> ScanGeometryC:
     return ScanGeometryC(scan.n_channels, scan.n_rows, scan.n_views, scan.views_per_rotation,
                          scan.channel_spacing, scan.row_spacing, scan.dsd, scan.r,
-                         scan.central_channel, scan.central_row, scan.angle0, scan.z0, scan.feed)
+                         scan.central_channel, scan.central_row, scan.angle0, scan.z0, scan.feed,
+                         scan.ffs_mode, 0, scan.anode_angle)
 
 
 def recon_struct(rc: S.ReconConfig) -> ReconConfigC:
@@ -273,7 +278,8 @@ class Plan:
         """The filtered buffer as a [view][row][channel_pitch] tensor."""
         import torch
         dt = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}[self.precision]
-        return filt.view(dt).view(self.scan.n_views, self.scan.n_rows, self.info.channel_pitch)
+        i = self.info
+        return filt.view(dt).view(i.n_parallel_views, i.n_filtered_rows, i.channel_pitch)
 
     # -- stages -------------------------------------------------------------
     def set_stream(self, stream=None):
@@ -313,7 +319,8 @@ class Plan:
     def alloc_lines(self):
         """FP32 [view][row][channel_pitch] buffer (filtered rows and their cotangents)."""
         import torch
-        return torch.empty((self.scan.n_views, self.scan.n_rows, self.info.channel_pitch),
+        i = self.info
+        return torch.empty((i.n_parallel_views, i.n_filtered_rows, i.channel_pitch),
                            dtype=torch.float32, device=self._dev())
 
     def backproject_adjoint(self, native_bar, lines_bar):

@@ -58,7 +58,23 @@ typedef struct {
   double angle0;               /* getTubeAngle(0) */
   double z0;                   /* getTablePosition(0) */
   double feed;                 /* getTablePosition(ppr) - getTablePosition(0) */
+  /* Flying focal spot (getFlyingFocalSpotMode, RawDataSet.h:102): the focal
+   * state of projection v is v mod S (S = 2, or 4 for phi+z) with the
+   * deflections and order of fct_icd's generate_system_matrix_ffs.cpp:66-67,
+   * 242-290.  With FFS the plan works on the interleaved detector
+   * (fct_plan_info.n_filtered_*): phi doubles the channels, z the rows, and
+   * there are n_views / S parallel views. */
+  int32_t ffs_mode;            /* fct_ffs_mode */
+  int32_t ffs_reserved;
+  double anode_angle;          /* rad; z-FFS deflection dr tan(anode) */
 } fct_scan_geometry;
+
+typedef enum {
+  FCT_FFS_NONE = 0,
+  FCT_FFS_PHI = 1,             /* in-plane (anti-radial) focal spot wobble */
+  FCT_FFS_Z = 2,               /* z (radial + axial) wobble */
+  FCT_FFS_PHI_Z = 3            /* both: a 4-state cycle */
+} fct_ffs_mode;
 
 /* Recon configuration: the keys of parse_config.cpp:39-57 plus filter
  * selection.  recon_kernel, tube_angle_offset and adaptive_filtration_s are
@@ -106,6 +122,12 @@ typedef struct {
   double z_rot;
   int32_t tile_z;
   int32_t bp_variant;
+  /* Filtered (parallel, interleaved) dimensions: equal to n_views, n_rows,
+   * n_channels without FFS. */
+  int32_t n_parallel_views;
+  int32_t n_filtered_rows;
+  int32_t n_filtered_channels;
+  int32_t ffs_states;          /* focal states S (1 without FFS) */
 } fct_plan_info;
 
 typedef struct {

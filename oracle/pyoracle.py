@@ -22,7 +22,9 @@ class OrScan(C.Structure):
                 ("channel_spacing", C.c_double), ("row_spacing", C.c_double),
                 ("dsd", C.c_double), ("r", C.c_double),
                 ("central_channel", C.c_double), ("central_row", C.c_double),
-                ("angle0", C.c_double), ("z0", C.c_double), ("feed", C.c_double)]
+                ("angle0", C.c_double), ("z0", C.c_double), ("feed", C.c_double),
+                ("ffs_mode", C.c_int32), ("ffs_reserved", C.c_int32),
+                ("anode_angle", C.c_double)]
 
 
 class OrRecon(C.Structure):
@@ -66,6 +68,7 @@ def lib():
         dp, fp, i64p, i32p = P(C.c_double), P(C.c_float), P(C.c_int64), P(C.c_int32)
         sig = {
             "or_geometry": (C.c_int, [P(OrScan), P(OrRecon), P(OrDerived)]),
+            "or_effective_scan": (None, [P(OrScan), P(OrScan)]),
             "or_slice_grid": (C.c_int, [C.c_double, C.c_double, C.c_double, dp, C.c_int]),
             "or_filter_kernel": (None, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, dp]),
             "or_rebin": (None, [P(OrScan), fp, dp, C.c_int, C.c_int]),
@@ -105,7 +108,22 @@ def _ptr(a: np.ndarray, ctype):
 def scan_struct(scan) -> OrScan:
     return OrScan(scan.n_channels, scan.n_rows, scan.n_views, scan.views_per_rotation,
                   scan.channel_spacing, scan.row_spacing, scan.dsd, scan.r,
-                  scan.central_channel, scan.central_row, scan.angle0, scan.z0, scan.feed)
+                  scan.central_channel, scan.central_row, scan.angle0, scan.z0, scan.feed,
+                  getattr(scan, "ffs_mode", 0), 0, getattr(scan, "anode_angle", 0.0))
+
+
+def _eff(scan):
+    """Parallel-domain functions work on the interleaved detector of an FFS
+    scan (ScanGeometry.effective(); checked against or_effective_scan)."""
+    return scan.effective() if getattr(scan, "ffs_mode", 0) else scan
+
+
+def effective_scan(scan) -> dict:
+    """The interleaved geometry as computed by the C++ restatement."""
+    out = OrScan()
+    s = scan_struct(scan)
+    lib().or_effective_scan(C.byref(s), C.byref(out))
+    return {k: getattr(out, k) for k, _ in OrScan._fields_}
 
 
 def recon_struct(rc) -> OrRecon:
@@ -116,7 +134,7 @@ def recon_struct(rc) -> OrRecon:
 
 def geometry(scan, rc=None) -> OrDerived:
     d = OrDerived()
-    s = scan_struct(scan)
+    s = scan_struct(_eff(scan))
     r = recon_struct(rc) if rc is not None else None
     lib().or_geometry(C.byref(s), C.byref(r) if r is not None else None, C.byref(d))
     return d
@@ -136,9 +154,12 @@ def filter_kernel(n, ds, kind=0, c=1.0, a=1.0) -> np.ndarray:
 
 
 def rebin(scan, raw: np.ndarray, v0=0, v1=None) -> np.ndarray:
-    v1 = scan.n_views if v1 is None else v1
+    """Raw fan data of `scan` -> parallel views [v0, v1) of its interleaved
+    detector."""
+    e = _eff(scan)
+    v1 = e.n_views if v1 is None else v1
     raw = np.ascontiguousarray(raw, np.float32)
-    out = np.zeros((v1 - v0, scan.n_rows, scan.n_channels), np.float64)
+    out = np.zeros((v1 - v0, e.n_rows, e.n_channels), np.float64)
     s = scan_struct(scan)
     lib().or_rebin(C.byref(s), _ptr(raw, C.c_float), _ptr(out, C.c_double), v0, v1)
     return out
@@ -158,7 +179,7 @@ def backproject(scan, rc, filtered: np.ndarray, v0: int, s0: int, s1: int):
     v1 = v0 + filtered.shape[0]
     vol = np.zeros((s1 - s0, rc.ny, rc.nx), np.float64)
     n = C.c_int64(0)
-    s, r = scan_struct(scan), recon_struct(rc)
+    s, r = scan_struct(_eff(scan)), recon_struct(rc)
     rc_ = lib().or_backproject(C.byref(s), C.byref(r), _ptr(filtered, C.c_double), v0, v1,
                                s0, s1, _ptr(vol, C.c_double), C.byref(n))
     if rc_ != 0:
@@ -171,7 +192,7 @@ def backproject_f32(scan, rc, filtered: np.ndarray, v0: int, s0: int, s1: int):
     v1 = v0 + filtered.shape[0]
     vol = np.zeros((s1 - s0, rc.ny, rc.nx), np.float32)
     n = C.c_int64(0)
-    s, r = scan_struct(scan), recon_struct(rc)
+    s, r = scan_struct(_eff(scan)), recon_struct(rc)
     rc_ = lib().or_backproject_f32(C.byref(s), C.byref(r), _ptr(filtered, C.c_float), v0,
                                    v1, s0, s1, _ptr(vol, C.c_float), C.byref(n))
     if rc_ != 0:
@@ -183,7 +204,7 @@ def thicken(scan, rc, native: np.ndarray) -> np.ndarray:
     d = geometry(scan, rc)
     native = np.ascontiguousarray(native, np.float64)
     out = np.zeros((d.n_slices_requested, rc.ny, rc.nx), np.float64)
-    s, r = scan_struct(scan), recon_struct(rc)
+    s, r = scan_struct(_eff(scan)), recon_struct(rc)
     lib().or_thicken(C.byref(s), C.byref(r), _ptr(native, C.c_double), _ptr(out, C.c_double))
     return out
 
@@ -191,7 +212,7 @@ def thicken(scan, rc, native: np.ndarray) -> np.ndarray:
 def bp_counts(scan, rc, s0, s1):
     counts = np.zeros((s1 - s0, rc.ny, rc.nx), np.int32)
     n = C.c_int64(0)
-    s, r = scan_struct(scan), recon_struct(rc)
+    s, r = scan_struct(_eff(scan)), recon_struct(rc)
     lib().or_bp_counts(C.byref(s), C.byref(r), s0, s1, _ptr(counts, C.c_int32), C.byref(n))
     return counts, n.value
 
@@ -199,7 +220,7 @@ def bp_counts(scan, rc, s0, s1):
 def bp_coverage(scan, rc, s0, s1) -> np.ndarray:
     """Per native voxel: min over theta~ of the weight sum of selected views."""
     out = np.zeros((s1 - s0, rc.ny, rc.nx), np.float64)
-    s, r = scan_struct(scan), recon_struct(rc)
+    s, r = scan_struct(_eff(scan)), recon_struct(rc)
     lib().or_bp_coverage(C.byref(s), C.byref(r), s0, s1, _ptr(out, C.c_double))
     return out
 
@@ -220,8 +241,8 @@ def covered_mask(scan, rc, threshold=0.5) -> np.ndarray:
 
 
 def bp_selection(scan, rc, ix, iy, iz) -> np.ndarray:
-    sel = np.zeros(scan.n_views, np.int32)
-    s, r = scan_struct(scan), recon_struct(rc)
+    sel = np.zeros(_eff(scan).n_views, np.int32)
+    s, r = scan_struct(_eff(scan)), recon_struct(rc)
     lib().or_bp_selection(C.byref(s), C.byref(r), ix, iy, iz, _ptr(sel, C.c_int32))
     return sel
 
@@ -248,7 +269,7 @@ def project_phantom(scan, phantom, v0=0, v1=None) -> np.ndarray:
 
 def view_range(scan, rc, s0, s1):
     a, b = C.c_int(0), C.c_int(0)
-    s, r = scan_struct(scan), recon_struct(rc)
+    s, r = scan_struct(_eff(scan)), recon_struct(rc)
     lib().or_view_range(C.byref(s), C.byref(r), s0, s1, C.byref(a), C.byref(b))
     return a.value, b.value
 
@@ -268,8 +289,9 @@ def reconstruct_native(scan, rc, raw: np.ndarray, s0: int, s1: int):
 def backproject_adjoint(scan, rc, xbar: np.ndarray, v0: int, v1: int, s0: int) -> np.ndarray:
     xbar = np.ascontiguousarray(xbar, np.float64)
     s1 = s0 + xbar.shape[0]
-    g = np.zeros((v1 - v0, scan.n_rows, scan.n_channels), np.float64)
-    s, r = scan_struct(scan), recon_struct(rc)
+    e = _eff(scan)
+    g = np.zeros((v1 - v0, e.n_rows, e.n_channels), np.float64)
+    s, r = scan_struct(e), recon_struct(rc)
     st = lib().or_backproject_adjoint(C.byref(s), C.byref(r), _ptr(xbar, C.c_double), v0, v1,
                                       s0, s1, _ptr(g, C.c_double))
     if st:

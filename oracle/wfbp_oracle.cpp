@@ -14,6 +14,7 @@
 #include <cstring>
 #include <vector>
 
+#include "../freect_amd/csrc/ffs_geometry.h"
 #include "../freect_amd/csrc/wfbp_math.h"
 
 namespace {
@@ -47,6 +48,28 @@ Geo make_geo(const or_scan* s) {
                               s->row_spacing, s->dsd, s->r, s->central_channel,
                               s->central_row, s->feed);
   return o;
+}
+
+fct_ffs::Geometry ffs_geometry(const or_scan* s) {
+  return fct_ffs::make_geometry(s->ffs_mode, s->n_channels, s->n_rows, s->n_views,
+                                s->views_per_rotation, s->channel_spacing, s->row_spacing,
+                                s->central_channel, s->central_row, s->r, s->dsd,
+                                s->anode_angle);
+}
+
+or_scan effective(const or_scan* s) {
+  const fct_ffs::Geometry F = ffs_geometry(s);
+  or_scan e = *s;
+  e.n_channels = F.e_ncols;
+  e.n_rows = F.e_nrows;
+  e.n_views = F.e_nviews;
+  e.views_per_rotation = F.e_ppr;
+  e.channel_spacing = F.e_dch;
+  e.row_spacing = F.e_drow;
+  e.central_channel = F.e_cc;
+  e.central_row = F.e_cr;
+  e.ffs_mode = 0;
+  return e;
 }
 
 // r(t) of ReconstructionMachine.cu:493-498 / ramp.m:12-15.
@@ -235,6 +258,8 @@ int bp_core(const or_scan* s, const or_recon* r, const P* filtered, int v0,
 
 extern "C" {
 
+void or_effective_scan(const or_scan* s, or_scan* out) { *out = effective(s); }
+
 int or_geometry(const or_scan* s, const or_recon* r, or_derived* d) {
   if (!s || !d) return -1;
   const Geo G = make_geo(s);
@@ -285,23 +310,31 @@ void or_filter_kernel(int n, double ds, int kind, double c, double a, double* h)
 }
 
 void or_rebin(const or_scan* s, const float* raw, double* out, int v0, int v1) {
-  const Geo G = make_geo(s);
-  std::vector<fct_tables::RebinEntry> tab(G.ncols);
-  for (int c = 0; c < G.ncols; ++c)
-    tab[c] = fct_tables::rebin_entry(c, s->central_channel, G.dgamma, G.dtheta, G.ncols);
-  const size_t vs = (size_t)G.nrows * G.ncols;
+  // Raw fan data [view][row][ch] of the scan -> parallel rows of the
+  // interleaved detector [j - v0][q][k].  Each entry names the focal state,
+  // its projection m = j + aoff (+1, clamped like the texture) and the fan
+  // channels; without FFS it is fct_tables::rebin_entry and the projection is
+  // the view.
+  const fct_ffs::Geometry F = ffs_geometry(s);
+  const int zpar = fct_ffs::has_z(F.mode) ? 2 : 1;
+  std::vector<fct_ffs::Entry> tab((size_t)zpar * F.e_ncols);
+  for (int zb = 0; zb < zpar; ++zb)
+    for (int k = 0; k < F.e_ncols; ++k) tab[(size_t)zb * F.e_ncols + k] = fct_ffs::rebin_entry(F, k, zb);
+  const size_t rvs = (size_t)F.nrows * F.ncols;        // raw view stride
+  const size_t evs = (size_t)F.e_nrows * F.e_ncols;    // parallel view stride
 #pragma omp parallel for schedule(static)
   for (int j = v0; j < v1; ++j) {
-    for (int row = 0; row < G.nrows; ++row) {
-      double* o = out + (size_t)(j - v0) * vs + (size_t)row * G.ncols;
-      for (int c = 0; c < G.ncols; ++c) {
-        const fct_tables::RebinEntry& e = tab[c];
-        if (!e.valid) { o[c] = 0.0; continue; }          // K:32-34
-        const int va = std::min(std::max(j + e.aoff, 0), G.nviews - 1);   // texture clamp
-        const int vb = std::min(std::max(j + e.aoff + 1, 0), G.nviews - 1);
-        const float* ra = raw + (size_t)va * vs + (size_t)row * G.ncols;
-        const float* rb = raw + (size_t)vb * vs + (size_t)row * G.ncols;
-        o[c] = (1.0 - e.fb) * ((1.0 - e.fa) * ra[e.b0] + e.fa * rb[e.b0]) +
+    for (int q = 0; q < F.e_nrows; ++q) {
+      const int zb = zpar == 2 ? (q & 1) : 0, row = zpar == 2 ? q >> 1 : q;
+      double* o = out + (size_t)(j - v0) * evs + (size_t)q * F.e_ncols;
+      for (int k = 0; k < F.e_ncols; ++k) {
+        const fct_ffs::Entry& e = tab[(size_t)zb * F.e_ncols + k];
+        if (!e.valid) { o[k] = 0.0; continue; }          // K:32-34
+        const int ma = std::min(std::max(j + e.aoff, 0), e.mmax);       // texture clamp
+        const int mb = std::min(std::max(j + e.aoff + 1, 0), e.mmax);
+        const float* ra = raw + (size_t)(ma * F.S + e.rho) * rvs + (size_t)row * F.ncols;
+        const float* rb = raw + (size_t)(mb * F.S + e.rho) * rvs + (size_t)row * F.ncols;
+        o[k] = (1.0 - e.fb) * ((1.0 - e.fa) * ra[e.b0] + e.fa * rb[e.b0]) +
                e.fb * ((1.0 - e.fa) * ra[e.b1] + e.fa * rb[e.b1]);
       }
     }
@@ -434,21 +467,30 @@ int or_bp_selection(const or_scan* s, const or_recon* r, int ix, int iy, int is,
 void or_project_phantom(const or_scan* s, double cyl_radius, double cyl_mu,
                         const or_ellipsoid* e, int ne, float* out, int v0, int v1) {
   const Geo G = make_geo(s);
+  const fct_ffs::Geometry F = ffs_geometry(s);
   const size_t vs = (size_t)G.nrows * G.ncols;
 #pragma omp parallel for schedule(static)
   for (int v = v0; v < v1; ++v) {
     const double a = s->angle0 + G.dtheta * (double)v;
-    const double sx = s->r * std::cos(a), sy = s->r * std::sin(a);
-    const double sz = s->z0 + G.dz * (double)v;
+    // Flying focal spot: state v mod S moves the focal spot by (dr, da, dz)
+    // along (e_w, e_u, e_z); the detector stays where the undeflected source
+    // puts it (generate_system_matrix_ffs.cpp:126-149).
+    const fct_ffs::Deflection fd =
+        fct_ffs::state_offset(F.mode, v % F.S, F.dr, F.da, F.anode);
+    const double ox = fd.dr * std::cos(a) - fd.da * std::sin(a);
+    const double oy = fd.dr * std::sin(a) + fd.da * std::cos(a);
+    const double sx = s->r * std::cos(a) + ox, sy = s->r * std::sin(a) + oy;
+    const double sz = s->z0 + G.dz * (double)v + fd.dz;
     for (int row = 0; row < G.nrows; ++row) {
       const double w = ((double)row - s->central_row) * s->row_spacing;
       for (int c = 0; c < G.ncols; ++c) {
         const double beta = ((double)c - s->central_channel) * G.dgamma;
-        // Ray from the source to its detector cell, unit direction u.
-        const double dx = -s->dsd * std::cos(a + beta);
-        const double dy = -s->dsd * std::sin(a + beta);
-        const double len = std::sqrt(s->dsd * s->dsd + w * w);
-        const double ux = dx / len, uy = dy / len, uz = w / len;
+        // Ray from the (deflected) source to its detector cell, unit direction u.
+        const double dx = -s->dsd * std::cos(a + beta) - ox;
+        const double dy = -s->dsd * std::sin(a + beta) - oy;
+        const double dzr = w - fd.dz;
+        const double len = std::sqrt(dx * dx + dy * dy + dzr * dzr);
+        const double ux = dx / len, uy = dy / len, uz = dzr / len;
         double acc = 0.0;
         {  // infinite water cylinder along z
           const double A = ux * ux + uy * uy;
@@ -497,14 +539,16 @@ void or_view_range(const or_scan* s, const or_recon* r, int s0, int s1, int* v0,
   *v1 = std::min(G.nviews, hi_v);
 }
 
-int or_reconstruct_native(const or_scan* s, const or_recon* r, const float* raw,
+int or_reconstruct_native(const or_scan* raw_scan, const or_recon* r, const float* raw,
                           int s0, int s1, double* vol, int64_t* n_selected) {
+  const or_scan es = effective(raw_scan);
+  const or_scan* s = &es;
   const Geo G = make_geo(s);
   int v0, v1;
   or_view_range(s, r, s0, s1, &v0, &v1);
   const size_t vs = (size_t)G.nrows * G.ncols;
   std::vector<double> par((size_t)(v1 - v0) * vs);
-  or_rebin(s, raw, par.data(), v0, v1);
+  or_rebin(raw_scan, raw, par.data(), v0, v1);
   or_derived d;
   or_geometry(s, r, &d);
   std::vector<double> h(d.nfft);

@@ -8,7 +8,9 @@
 //   or_geometry        ReconstructionMachine.cu:44-74, recon_structs.h:69-83
 //   or_slice_grid      ReconstructionMachine.cu:107-137
 //   or_filter_kernel   ReconstructionMachine.cu:478-513, resources/ramp.m:1-15
-//   or_rebin           cuda_kernels.cuh:11-39
+//   or_rebin           cuda_kernels.cuh:11-39; FFS interleave of
+//                      freect_amd/csrc/ffs_geometry.h (deflections and order
+//                      of fct_icd generate_system_matrix_ffs.cpp:66-67, 242-290)
 //   or_filter_rows     cuda_kernels.cuh:41-77 + ReconstructionMachine.cu:215-290
 //                      (FFT product == linear convolution, see below)
 //   or_backproject     cuda_kernels.cuh:79-93, 95-196, 198-287
@@ -36,6 +38,9 @@ typedef struct {
   double angle0;            // tube angle of view 0 (rad)
   double z0;                // source z of view 0 (mm)
   double feed;              // table feed per rotation (mm, signed)
+  int32_t ffs_mode;         // 0 none, 1 phi, 2 z, 3 phi+z (focal state = view mod S)
+  int32_t ffs_reserved;
+  double anode_angle;       // rad
 } or_scan;
 
 typedef struct {
@@ -62,6 +67,11 @@ typedef struct {
   double dmu;        // additive attenuation (1/mm)
 } or_ellipsoid;
 
+// The interleaved detector an FFS scan is reconstructed on (ffs_mode 0);
+// the scan itself without FFS.  Every function below except or_rebin,
+// or_rebin_adjoint, or_project_phantom, or_reconstruct_native and
+// or_adjoint_native takes the interleaved (parallel-domain) geometry.
+void or_effective_scan(const or_scan* s, or_scan* out);
 int or_geometry(const or_scan* s, const or_recon* r, or_derived* d);
 int or_slice_grid(double start, double end, double step, double* z, int cap);
 void or_filter_kernel(int n, double ds, int kind, double c, double a, double* h);

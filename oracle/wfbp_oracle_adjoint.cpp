@@ -15,6 +15,7 @@
 #include <cmath>
 #include <vector>
 
+#include "../freect_amd/csrc/ffs_geometry.h"
 #include "../freect_amd/csrc/wfbp_math.h"
 #include "wfbp_oracle.h"
 
@@ -136,37 +137,49 @@ int or_backproject_adjoint(const or_scan* s, const or_recon* r, const double* xb
 // Transpose of or_rebin for parallel views [v0, v1): pbar is the whole raw
 // array [view][row][ch] and is accumulated into (caller zeroes it).
 void or_rebin_adjoint(const or_scan* s, const double* gbar, double* pbar, int v0, int v1) {
-  const int ncols = s->n_channels, nrows = s->n_rows, nviews = s->n_views;
-  const double dgamma = s->channel_spacing / s->dsd;
-  const double dtheta = 2.0 * kPi / s->views_per_rotation;
-  std::vector<fct_tables::RebinEntry> tab(ncols);
-  for (int c = 0; c < ncols; ++c)
-    tab[c] = fct_tables::rebin_entry(c, s->central_channel, dgamma, dtheta, ncols);
-  const size_t vs = (size_t)nrows * ncols;
+  // Transpose of or_rebin: parallel rows [j - v0][q][k] of the interleaved
+  // detector scatter into raw fan data [view][row][ch].  Raw rows are split
+  // over threads; both effective rows 2r, 2r+1 of raw row r stay in one.
+  const fct_ffs::Geometry F = fct_ffs::make_geometry(
+      s->ffs_mode, s->n_channels, s->n_rows, s->n_views, s->views_per_rotation,
+      s->channel_spacing, s->row_spacing, s->central_channel, s->central_row, s->r, s->dsd,
+      s->anode_angle);
+  const int zpar = fct_ffs::has_z(F.mode) ? 2 : 1;
+  std::vector<fct_ffs::Entry> tab((size_t)zpar * F.e_ncols);
+  for (int zb = 0; zb < zpar; ++zb)
+    for (int k = 0; k < F.e_ncols; ++k) tab[(size_t)zb * F.e_ncols + k] = fct_ffs::rebin_entry(F, k, zb);
+  const size_t rvs = (size_t)F.nrows * F.ncols;
+  const size_t evs = (size_t)F.e_nrows * F.e_ncols;
 #pragma omp parallel for schedule(static)
-  for (int row = 0; row < nrows; ++row) {
-    for (int j = v0; j < v1; ++j) {
-      const double* gl = gbar + (size_t)(j - v0) * vs + (size_t)row * ncols;
-      for (int c = 0; c < ncols; ++c) {
-        const fct_tables::RebinEntry& e = tab[c];
-        if (!e.valid) continue;
-        const int va = std::min(std::max(j + e.aoff, 0), nviews - 1);
-        const int vb = std::min(std::max(j + e.aoff + 1, 0), nviews - 1);
-        double* pa = pbar + (size_t)va * vs + (size_t)row * ncols;
-        double* pb = pbar + (size_t)vb * vs + (size_t)row * ncols;
-        const double y = gl[c];
-        pa[e.b0] += (1 - e.fb) * (1 - e.fa) * y;
-        pb[e.b0] += (1 - e.fb) * e.fa * y;
-        pa[e.b1] += e.fb * (1 - e.fa) * y;
-        pb[e.b1] += e.fb * e.fa * y;
+  for (int row = 0; row < F.nrows; ++row) {
+    for (int zb = 0; zb < zpar; ++zb) {
+      const int q = zpar == 2 ? 2 * row + zb : row;
+      for (int j = v0; j < v1; ++j) {
+        const double* gl = gbar + (size_t)(j - v0) * evs + (size_t)q * F.e_ncols;
+        for (int k = 0; k < F.e_ncols; ++k) {
+          const fct_ffs::Entry& e = tab[(size_t)zb * F.e_ncols + k];
+          if (!e.valid) continue;
+          const int va = std::min(std::max(j + e.aoff, 0), e.mmax) * F.S + e.rho;
+          const int vb = std::min(std::max(j + e.aoff + 1, 0), e.mmax) * F.S + e.rho;
+          double* pa = pbar + (size_t)va * rvs + (size_t)row * F.ncols;
+          double* pb = pbar + (size_t)vb * rvs + (size_t)row * F.ncols;
+          const double y = gl[k];
+          pa[e.b0] += (1 - e.fb) * (1 - e.fa) * y;
+          pb[e.b0] += (1 - e.fb) * e.fa * y;
+          pa[e.b1] += e.fb * (1 - e.fa) * y;
+          pb[e.b1] += e.fb * e.fa * y;
+        }
       }
     }
   }
 }
 
 // Whole transpose for native slices [s0, s1): xbar -> pbar (all raw views).
-int or_adjoint_native(const or_scan* s, const or_recon* r, const double* xbar, int s0, int s1,
-                      double* pbar) {
+int or_adjoint_native(const or_scan* raw_scan, const or_recon* r, const double* xbar, int s0,
+                      int s1, double* pbar) {
+  or_scan es;
+  or_effective_scan(raw_scan, &es);
+  const or_scan* s = &es;
   int v0, v1;
   or_view_range(s, r, s0, s1, &v0, &v1);
   or_derived d;
@@ -178,8 +191,8 @@ int or_adjoint_native(const or_scan* s, const or_recon* r, const double* xbar, i
   std::vector<double> h(d.nfft);
   or_filter_kernel(d.nfft, d.ds, r->filter_kind, r->filter_c, r->filter_a, h.data());
   or_filter_rows(gbar.data(), (int64_t)(v1 - v0) * s->n_rows, s->n_channels, h.data(), d.nfft, 1);
-  std::fill(pbar, pbar + (size_t)s->n_views * vs, 0.0);
-  or_rebin_adjoint(s, gbar.data(), pbar, v0, v1);
+  std::fill(pbar, pbar + (size_t)raw_scan->n_views * raw_scan->n_rows * raw_scan->n_channels, 0.0);
+  or_rebin_adjoint(raw_scan, gbar.data(), pbar, v0, v1);
   return 0;
 }
 
