@@ -7,10 +7,17 @@ into 512 x 512 x 320 slices at 0.625 mm, 500 mm FOV, ramp filter.
 
 A step is one whole reconstruction (rebin + filter -> backprojection ->
 thickening) with the raw scan already resident in HBM.  The chosen path
-stores filtered projections as FP16 with a per-(view,row) FP32 scale and
-accumulates in FP32; the plain FP32 path is timed beside it.  value is
-voxel-views/s of the chosen path, where a voxel-view is a (native voxel,
-view) pair selected by the reconstruction rule (counted exactly on the GPU).
+stores filtered projections as FP16 with a per-(view,row) FP32 scale,
+accumulates in FP32 and runs as a Batch: the step recorded once into a
+hipGraph, the persistent backprojection, the tile depth from the autotuner.
+The plain FP32 single-plan path (and the FP16 single-plan path) are timed
+beside it.  value is voxel-views/s of the chosen path, where a voxel-view is
+a (native voxel, view) pair selected by the reconstruction rule (counted
+exactly on the GPU).
+
+--workload batch measures the 24 uneven jobs instead (aggregate
+voxel-views/s of one mixed-precision Batch against the same jobs run one by
+one on the plain FP32 path).
 """
 from __future__ import annotations
 
@@ -99,6 +106,252 @@ def cpu_baseline(w, raw_np, n_vv_total, n_native):
             "s_per_volume_est": t_total}, (s0, vol)
 
 
+def batch_path(ctx, jobs, raws, precision, steps, warmup, torch, candidates=(4, 8, 16)):
+    """Chosen path: one Batch (per-job rebin+filter, one persistent
+    backprojection over every job's tiles, per-job thicken) recorded into a
+    hipGraph and replayed once per step, tile depth from the autotuner.
+    Returns ms/step, the persistent backprojection's mean ms (HIP events
+    around BP-only launches on the batch stream), the tuner's table and the
+    volumes."""
+    from freect_amd import wfbp
+    stream = torch.cuda.Stream()
+    b = wfbp.Batch(ctx, jobs, precision=precision, stream=stream)
+    vols = [p.alloc_volume() for p in b.plans]
+    for j, (r, v) in enumerate(zip(raws, vols)):
+        b.submit(j, r, v)
+    torch.cuda.synchronize()
+    tuned = b.autotune(candidates, reps=2)
+    log("autotune", precision, tuned, "->", b.info()["tile_z"])
+    for _ in range(max(warmup, 1)):
+        b.run(wfbp.Batch.GRAPH)
+    b.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(steps):
+        b.run(wfbp.Batch.GRAPH)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = ev[0].elapsed_time(ev[1]) / steps
+    wall_ms = (t1 - t0) * 1e3 / steps
+    # The persistent backprojection alone, same stream, same buffers.
+    bev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    for a, c in bev:
+        a.record(stream)
+        b.backproject()
+        c.record(stream)
+    torch.cuda.synchronize()
+    bp_ms = sum(a.elapsed_time(c) for a, c in bev) / steps
+    # Restore the volumes of a full run (the BP-only launches rewrote native).
+    b.run(wfbp.Batch.GRAPH)
+    b.synchronize()
+    return max(ms, wall_ms), bp_ms, tuned, b, vols
+
+
+def _cores():
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    return cores
+
+
+def bench_batch(args, torch):
+    """24 uneven jobs on the 888x64 detector: sequential plain FP32 single-plan
+    reconstructions vs one mixed-precision Batch (graph + persistent BP)."""
+    import numpy as np
+    from freect_amd import scans as S
+    from freect_amd import wfbp
+    ctx = wfbp.Context(0)
+    ws = S.batch_jobs()
+    raws = []
+    for w in ws:
+        raws.append(ctx.phantom_project(w.scan, w.phantom, w.i0, w.seed))
+    torch.cuda.synchronize()
+    plans = [wfbp.Plan(ctx, w.scan, w.recon, precision="fp32") for w in ws]
+    n_vv = sum(p.count_voxel_views() for p in plans)
+    log("batch: jobs", len(ws), "voxel-views", n_vv)
+    vols = [p.alloc_volume() for p in plans]
+
+    def seq():
+        for p, r, v in zip(plans, raws, vols):
+            p.reconstruct(r, v)
+
+    for _ in range(max(args.warmup, 1)):
+        seq()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        seq()
+    torch.cuda.synchronize()
+    seq_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log("sequential fp32 ms", seq_ms)
+    jobs = [(w.scan, w.recon) for w in ws]
+    ms, bp_ms, tuned, b, bvols = batch_path(ctx, jobs, raws, args.precision, args.steps,
+                                            args.warmup, torch)
+    log("batched", args.precision, "ms", ms, "bp ms", bp_ms)
+    # Per-job HU difference between the two paths (FP16 storage vs FP32).
+    hu = []
+    for v0, v1 in zip(vols, bvols):
+        d = (v1.double() - v0.double()) / (S.MU_WATER / 1000.0)
+        hu.append(float(d.pow(2).mean().sqrt()))
+    bpv = BYTES_PER_VV[args.precision]
+    out = {
+        "metric": "aggregate voxel-views/s, 24 uneven jobs, batched mixed vs sequential plain FP32",
+        "value": n_vv / (ms * 1e-3), "unit": "voxel-views/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "storage": args.precision,
+        "data": "synthetic (24 seeded jobs, Poisson noise, generated on the GPU)",
+        "config": {"workload": "batch: 24 jobs on 888x64, 2-20 rotations, pitch "
+                               "{0.516,0.984,1.375}, 256^2/512^2, 16-240 slices",
+                   "voxel_views": n_vv, "tile_z": b.info()["tile_z"], "autotune_ms": tuned},
+        "sequential_plain_fp32": {"ms_per_step": seq_ms,
+                                  "voxel_views_per_s": n_vv / (seq_ms * 1e-3)},
+        "speedup_vs_sequential_plain_fp32": seq_ms / ms,
+        "roofline": {"bound": "hbm", "kernel": "bp_batch_kernel",
+                     "achieved": n_vv * bpv / (bp_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": n_vv * bpv / (bp_ms * 1e-3) / HBM_PEAK,
+                     "traffic": None, "bp_ms": bp_ms, "bytes_per_voxel_view": bpv},
+        "hu_rmse_batched_vs_sequential_per_job_max": max(hu),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args, torch):
+    """Differentiable WFBP on the adjoint workload (888x64, 3 rotations,
+    512^2 x 64 slices, noiseless): WFBP volume x0, then 10 gradient steps on
+    0.5 ||P x - p||^2 (P the hand-written adjoint, step 1 / lambda_max from
+    power iteration), keeping the sinogram-sized residual ("store") or
+    recomputing it in the backward pass ("recompute"); HU error of every
+    iterate against the noiseless phantom on the native grid."""
+    import numpy as np
+    from freect_amd import autograd as AG
+    from freect_amd import scans as S
+    from freect_amd import wfbp
+    w = S.adjoint()
+    sc, rc = w.scan, w.recon
+    ctx = wfbp.Context(0)
+    raw = ctx.phantom_project(sc, w.phantom, 0.0, w.seed)
+    plan = wfbp.Plan(ctx, sc, rc, precision="fp32")
+    x0 = AG.wfbp(raw, plan).detach()
+    torch.cuda.synchronize()
+    info = plan.info
+    n = info.n_slices_native
+    # Truth on the native grid: central slices (fully covered by the helix),
+    # voxels inside 0.9 of the acquisition FOV.
+    csw = info.collimated_slice_width
+    zs = [rc.start_pos + csw * i for i in range(n)]
+    sl = list(range(n // 4, 3 * n // 4, 4))
+    xy = (np.arange(rc.nx) - 0.5 * (rc.nx - 1)) * rc.recon_fov / rc.nx
+    X, Y = np.meshgrid(xy, xy)
+    fov = X * X + Y * Y <= (0.45 * info.acquisition_fov) ** 2
+    truth = {i: w.phantom.mu_at(X, Y, zs[i]) for i in sl}
+    HU = S.MU_WATER / 1000.0
+
+    def hu_rmse(x):
+        xs = x.detach().cpu().numpy()
+        d = np.concatenate([((xs[i] - truth[i]) / HU)[fov] for i in sl])
+        return float(np.sqrt(np.mean(d * d)))
+
+    lam = AG.power_iteration(plan, x0.shape)
+    out_modes = {}
+    for mode in ("store", "recompute"):
+        hist = []
+        x, st = AG.refine(plan, raw, x0, steps=10, mode=mode, lam=lam,
+                          callback=lambda it, xx: hist.append(hu_rmse(xx)))
+        out_modes[mode] = {"hu_rmse_per_step": hist, "loss": st["loss"],
+                           "ms_per_step": float(np.median(st["ms_per_step"])),
+                           "peak_mb": float(max(st["peak_mb"]))}
+        log(mode, out_modes[mode])
+    out = {
+        "metric": "differentiable WFBP: 10 data-consistency steps, HU RMSE vs noiseless phantom",
+        "value": out_modes["store"]["hu_rmse_per_step"][-1], "unit": "HU RMSE",
+        "n_gpus": 1, "steps": 10, "warmup": 0,
+        "ms_per_step": out_modes["store"]["ms_per_step"], "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, noiseless, generated on the GPU)",
+        "config": {"workload": "adjoint: 888x64, 2952 views (3 rotations), 512^2 x 64 slices",
+                   "lambda_max": lam, "hu_mask": "native slices n/4..3n/4 step 4, r < 0.45 FOV"},
+        "hu_rmse_wfbp": hu_rmse(x0),
+        "modes": out_modes,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_ffs(args, torch):
+    """Flying-focal-spot scan (Siemens-like, phi + z FFS, 2304 projections per
+    rotation interleaved to 1472 x 64): plain FP32 and mixed precision, HU
+    error of both against the FP64 restatement on the central native slice
+    (computed in the CPU-baseline leg, which also times it)."""
+    import numpy as np
+    from freect_amd import scans as S
+    from freect_amd import wfbp
+    w = S.ffs()
+    sc, rc = w.scan, w.recon
+    ctx = wfbp.Context(0)
+    raw = ctx.phantom_project(sc, w.phantom, w.i0, w.seed)
+    torch.cuda.synchronize()
+    res, vols = {}, {}
+    n_vv = None
+    for prec in ("fp32", args.precision):
+        plan = wfbp.Plan(ctx, sc, rc, precision=prec)
+        if n_vv is None:
+            n_vv = plan.count_voxel_views()
+            info = plan.info
+        ms, bp_ms, vol = time_path(plan, raw, args.steps, args.warmup, torch)
+        res[prec] = {"ms_per_step": ms, "bp_ms": bp_ms, "voxel_views_per_s": n_vv / (ms * 1e-3)}
+        vols[prec] = (plan, vol)
+        log("ffs", prec, res[prec])
+    cpu, hu = None, {}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        s0 = info.n_slices_native // 2
+        t = time.perf_counter()
+        ref, n_sel = pyoracle.reconstruct_native(sc, rc, raw.cpu().numpy(), s0, s0 + 1)
+        t_cpu = time.perf_counter() - t
+        cpu = {"value": n_sel / t_cpu, "unit": "voxel-views/s", "cores": _cores(),
+               "kind": "port", "sample": f"FP64 restatement, rebin+filter+BP of native slice {s0}"}
+        xy = (np.arange(rc.nx) - 0.5 * (rc.nx - 1)) * rc.recon_fov / rc.nx
+        X, Y = np.meshgrid(xy, xy)
+        mask = X * X + Y * Y <= (0.5 * info.acquisition_fov) ** 2
+        for prec, (plan, _) in vols.items():
+            nat = plan.alloc_native()
+            filt, scale = plan.alloc_filtered()
+            plan.rebin_filter(raw, filt, scale)
+            plan.backproject(filt, scale, nat)
+            plan.synchronize()
+            d = (nat[s0].cpu().numpy().astype(np.float64) - ref[0]) / (S.MU_WATER / 1000.0)
+            hu[prec] = {"rmse": float(np.sqrt(np.mean(d[mask] ** 2))),
+                        "max": float(np.abs(d[mask]).max()), "native_slice": int(s0)}
+        log("ffs hu", hu)
+    chosen = res[args.precision]
+    bpv = BYTES_PER_VV[args.precision]
+    out = {
+        "metric": "FFS scan: voxel-views/s, mixed vs plain FP32, HU vs FP64",
+        "value": chosen["voxel_views_per_s"], "unit": "voxel-views/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": chosen["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "storage": args.precision,
+        "data": "synthetic (seeded ellipsoid phantom through phi+z FFS rays, Poisson I0=2.5e4)",
+        "config": {"workload": "ffs: Siemens-like 736x32, 2304 proj/rot phi+z FFS -> 1472x64, "
+                               "pitch 0.8, 8 rotations, 512^2 x 200 slices (1.0 mm at 0.5 mm)",
+                   "voxel_views": n_vv, "n_parallel_views": info.n_parallel_views},
+        "plain_fp32": res["fp32"], "speedup_vs_plain_fp32":
+            res["fp32"]["ms_per_step"] / chosen["ms_per_step"],
+        "roofline": {"bound": "hbm", "kernel": "bp_v1_kernel",
+                     "achieved": n_vv * bpv / (chosen["bp_ms"] * 1e-3) / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": n_vv * bpv / (chosen["bp_ms"] * 1e-3) / HBM_PEAK, "traffic": None},
+        "cpu_baseline": cpu, "hu_error_vs_fp64": hu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +363,12 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--workload", default="full", choices=["full", "batch", "adjoint", "ffs"],
+                    help="full: the headline scan (default); batch: 24 uneven jobs; adjoint: "
+                         "the refinement report")
+    ap.add_argument("--path", default="batch", choices=["batch", "plan"],
+                    help="chosen path: hipGraph-replayed Batch with the persistent "
+                         "backprojection (default) or the single-plan stage calls")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
@@ -130,6 +389,15 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
         del src, dst
+    if args.workload == "batch":
+        bench_batch(args, torch)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args, torch)
+        return
+    if args.workload == "ffs":
+        bench_ffs(args, torch)
+        return
     w = S.full()
     ctx = wfbp.Context(0)
     log("generating synthetic scan", w.scan.n_views, w.scan.n_rows, w.scan.n_channels)
@@ -154,7 +422,22 @@ def main():
         vols[prec] = vol
         log(prec, res[prec])
 
-    chosen = res[args.precision]
+    kernel = "bp_v1_kernel"
+    tuned = None
+    if args.path == "batch":
+        log("timing batched", args.precision)
+        ms, bp_ms, tuned, b, bvols = batch_path(ctx, [(w.scan, w.recon)], [raw], args.precision,
+                                                args.steps, args.warmup, torch)
+        res["batch"] = {"ms_per_step": ms, "bp_ms": bp_ms,
+                        "voxel_views_per_s": n_vv / (ms * 1e-3),
+                        "bp_voxel_views_per_s": n_vv / (bp_ms * 1e-3),
+                        "tile_z": b.info()["tile_z"], "autotune_ms": tuned}
+        vols["batch"] = bvols[0]
+        log("batch", res["batch"])
+        chosen = res["batch"]
+        kernel = "bp_batch_kernel"
+    else:
+        chosen = res[args.precision]
     info = p0.info
     cpu = None
     hu = {}
@@ -199,14 +482,16 @@ def main():
         "data": "synthetic (seeded ellipsoid phantom, Poisson noise I0=5e4, generated on the GPU)",
         "config": {"workload": "full: 888ch x 64 rows, 9840 views (984/rot, pitch 0.984), "
                                "512x512x320 slices at 0.625 mm, FOV 500 mm, ramp",
-                   "voxel_views": n_vv, "tile_z": info.tile_z},
+                   "voxel_views": n_vv, "path": args.path,
+                   "tile_z": chosen.get("tile_z", info.tile_z), "autotune_ms": tuned},
         "s_per_volume": chosen["ms_per_step"] * 1e-3,
         "plain_fp32": {"ms_per_step": res["fp32"]["ms_per_step"],
                        "s_per_volume": res["fp32"]["ms_per_step"] * 1e-3,
                        "voxel_views_per_s": res["fp32"]["voxel_views_per_s"],
                        "bp_ms": res["fp32"]["bp_ms"]},
         "speedup_vs_plain_fp32": res["fp32"]["ms_per_step"] / chosen["ms_per_step"],
-        "roofline": {"bound": "hbm", "kernel": "bp_kernel", "achieved": achieved / 1e9,
+        "paths": res,
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic, "bp_ms": chosen["bp_ms"],
                      "bytes_per_voxel_view": bpv},

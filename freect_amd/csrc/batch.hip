@@ -262,6 +262,12 @@ FCT_API int fct_wfbp_gpu_batch_run(fct_wfbp_gpu_batch* b, int32_t mode) {
   return FCT_OK;
 }
 
+FCT_API int fct_wfbp_gpu_batch_backproject(fct_wfbp_gpu_batch* b) {
+  if (!b) return FCT_ERR_INVALID_ARGUMENT;
+  BATCH_TRY(b, hipSetDevice(b->device));
+  return enqueue_backproject(b);
+}
+
 FCT_API int fct_wfbp_gpu_batch_synchronize(fct_wfbp_gpu_batch* b) {
   if (!b) return FCT_ERR_INVALID_ARGUMENT;
   BATCH_TRY(b, hipSetDevice(b->device));

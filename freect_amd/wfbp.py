@@ -104,6 +104,7 @@ SIGNATURES = {
     "fct_wfbp_gpu_batch_submit": (C.c_int, [_VP, C.c_int32, _VP, _VP]),
     "fct_wfbp_gpu_batch_run": (C.c_int, [_VP, C.c_int32]),
     "fct_wfbp_gpu_batch_synchronize": (C.c_int, [_VP]),
+    "fct_wfbp_gpu_batch_backproject": (C.c_int, [_VP]),
     "fct_wfbp_gpu_batch_autotune": (C.c_int, [_VP, _P(C.c_int32), C.c_int32, C.c_int32,
                                               _P(C.c_float)]),
     "fct_wfbp_gpu_batch_info": (C.c_int, [_VP, _P(BatchInfoC)]),
@@ -428,6 +429,10 @@ class Batch:
 
     def synchronize(self):
         self.ctx.check(load_library().fct_wfbp_gpu_batch_synchronize(self.h), "batch_sync")
+
+    def backproject(self):
+        """Only the persistent backprojection (eager), on the batch stream."""
+        self.ctx.check(load_library().fct_wfbp_gpu_batch_backproject(self.h), "batch_bp")
 
     def autotune(self, candidates=(4, 8, 16), reps: int = 2) -> dict:
         n = len(candidates)

@@ -268,6 +268,10 @@ FCT_API int fct_wfbp_gpu_batch_submit(fct_wfbp_gpu_batch* batch, int32_t job, co
 /* Enqueue one run on the batch stream (asynchronous). */
 FCT_API int fct_wfbp_gpu_batch_run(fct_wfbp_gpu_batch* batch, int32_t mode);
 FCT_API int fct_wfbp_gpu_batch_synchronize(fct_wfbp_gpu_batch* batch);
+/* Enqueue only the persistent backprojection of every job (eager), reading
+ * the filtered projections the last run left in the batch's buffers and
+ * rewriting the native volumes.  For timing the kernel on the batch stream. */
+FCT_API int fct_wfbp_gpu_batch_backproject(fct_wfbp_gpu_batch* batch);
 /* Autotuner: runs rebin_filter of every job once, then times the persistent
  * backprojection `reps` times for each candidate tile_z (HIP events on the
  * batch stream), keeps the fastest and returns each candidate's mean ms in
