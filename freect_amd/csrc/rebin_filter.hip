@@ -198,11 +198,15 @@ __device__ __forceinline__ void load_twiddles(FctCplx* out, const FctCplx* __res
                          : FctCplx{1.0f, 0.0f};
 }
 
-// Exchange ahead of pass P (P >= 1) through LDS.
-template <int N, int P, bool kPingPong>
+// Exchange ahead of pass P (P >= 1) through LDS.  With two buffers the
+// exchanges alternate between them in the order they happen (E counts the
+// exchanges before this one in the line pair; a pair has an even number), so
+// a store never targets the buffer the previous exchange is still loading
+// from and one barrier per exchange suffices.
+template <int N, int P, bool kPingPong, int E>
 __device__ __forceinline__ void fft_exchange(FctCplx* v, FctCplx* lds, int t) {
   constexpr int kSlots = N + N / 8;
-  FctCplx* buf = kPingPong ? lds + (P & 1) * kSlots : lds;
+  FctCplx* buf = kPingPong ? lds + (E & 1) * kSlots : lds;
   if (!kPingPong) __syncthreads();   // earlier loads from the single buffer are done
   fct_fft_pass_store<N, P - 1>(v, buf, t);
   __syncthreads();
@@ -211,17 +215,19 @@ __device__ __forceinline__ void fft_exchange(FctCplx* v, FctCplx* lds, int t) {
 
 // One forward transform of the 8 values per thread in v (natural-order
 // inputs t + m T in v[m]; outputs t + m T in v[fct_fft_final_reg<N>(m)]).
-template <int N, bool kPingPong>
+// E0: exchanges already made in this line pair (0 for the forward
+// transform, kPasses - 1 for the inverse).
+template <int N, bool kPingPong, int E0>
 __device__ __forceinline__ void fft_forward_regs(FctCplx* v, FctCplx* lds, int t,
                                                  const FctCplx* tw1, const FctCplx* tw2,
                                                  const FctCplx* tw3) {
   fct_fft_pass_compute<N, 0>(v, nullptr);
-  fft_exchange<N, 1, kPingPong>(v, lds, t);
+  fft_exchange<N, 1, kPingPong, E0>(v, lds, t);
   fct_fft_pass_compute<N, 1>(v, tw1);
-  fft_exchange<N, 2, kPingPong>(v, lds, t);
+  fft_exchange<N, 2, kPingPong, E0 + 1>(v, lds, t);
   fct_fft_pass_compute<N, 2>(v, tw2);
   if constexpr (FctFftShape<N>::kPasses == 4) {
-    fft_exchange<N, 3, kPingPong>(v, lds, t);
+    fft_exchange<N, 3, kPingPong, E0 + 2>(v, lds, t);
     fct_fft_pass_compute<N, 3>(v, tw3);
   }
 }
@@ -273,14 +279,14 @@ __global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs
       v[m] = FctCplx{va, vb};
       v[m + 4] = FctCplx{0.0f, 0.0f};
     }
-    fft_forward_regs<N, kPingPong>(v, lds, t, tw1, tw2, tw3);
+    fft_forward_regs<N, kPingPong, 0>(v, lds, t, tw1, tw2, tw3);
     FctCplx w[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const FctCplx z = v[fct_fft_final_reg<N>(m)];
       w[m] = FctCplx{z.x * hn[m], -z.y * hn[m]};   // conj(H/N * Z)
     }
-    fft_forward_regs<N, kPingPong>(w, lds, t, tw1, tw2, tw3);   // y = conj(result)
+    fft_forward_regs<N, kPingPong, S::kPasses - 1>(w, lds, t, tw1, tw2, tw3);   // y = conj(result)
 
     const size_t o0 = (size_t)l0 * a.cpitch, o1 = (size_t)l1 * a.cpitch;
     if (a.precision == FCT_PRECISION_FP32) {
