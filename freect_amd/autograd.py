@@ -2,14 +2,18 @@
 
 A   : raw fan data p -> native volume x (rebin -> filter -> W-weighted
       backprojection, the FP32 plan of wfbp.Plan).
-A^T : the exact transpose, i.e. a matched helical forward projector
-      (wfbp.Plan.adjoint, HIP kernels in csrc/adjoint.hip).
+A^T : its exact transpose (wfbp.Plan.adjoint, HIP kernels in csrc/adjoint.hip).
+P0  : R^T B^T, the same transposed kernels without the filter stage (B the
+      W-weighted backprojection, R the rebinning), a matched helical
+      projector; P0^T = B R is the forward chain without the filter.
 
-WFBPFunction.apply(p) = A p with backward g -> A^T g, and
-ProjectFunction.apply(x) = A^T x with backward g -> A g, so torch can
-differentiate any loss built on them.  `refine` runs gradient steps on
-||A^T x - p||^2 starting from the WFBP volume, with the step size 1 / ||A A^T||
-estimated by power iteration.
+WFBPFunction.apply(p) = A p with backward g -> A^T g; ProjectFunction.apply(x)
+= A^T x with backward g -> A g; MatchedProjectFunction.apply(x) = P0 x with
+backward g -> P0^T g.  `refine` runs gradient steps on 0.5 ||k P x - p||^2
+from the WFBP volume, P = P0 (default; k fitted by least squares at the start
+so that k P0 x0 matches p) or P = A^T (k = 1, the objective whose minimiser is
+not the attenuation map, since A^T carries the ramp filter), with step
+1 / (k^2 ||P P^T||) from power iteration.
 """
 from __future__ import annotations
 
@@ -50,6 +54,39 @@ class ProjectFunction(torch.autograd.Function):
         return x_bar, None
 
 
+def _p0(plan, x):
+    """P0 x = R^T B^T x (raw fan data)."""
+    lines = plan.alloc_lines()
+    plan.backproject_adjoint(x.contiguous(), lines)   # zero-fills, then scatters
+    p = plan.alloc_raw()
+    plan.rebin_adjoint(lines, p)
+    return p
+
+
+def _p0t(plan, g):
+    """P0^T g = B R g (native volume)."""
+    lines = plan.alloc_lines()
+    plan.rebin_lines(g.contiguous(), lines)
+    x = plan.alloc_native()
+    plan.backproject(lines, None, x)
+    return x
+
+
+class MatchedProjectFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, plan):
+        ctx.plan = plan
+        return _p0(plan, x)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _p0t(ctx.plan, g), None
+
+
+def matched_project(x, plan):
+    return MatchedProjectFunction.apply(x, plan)
+
+
 def wfbp(raw, plan):
     return WFBPFunction.apply(raw, plan)
 
@@ -58,41 +95,59 @@ def project(x, plan):
     return ProjectFunction.apply(x, plan)
 
 
-def power_iteration(plan, shape, iters: int = 12, seed: int = 0) -> float:
-    """Largest eigenvalue of A A^T (= ||A^T||^2) by power iteration on volumes."""
+def power_iteration(plan, shape, iters: int = 12, seed: int = 0,
+                    operator: str = "adjoint") -> float:
+    """Largest eigenvalue of P^T P for P = A^T ("adjoint") or P0 ("matched"),
+    by power iteration on volumes."""
     gen = torch.Generator(device="cuda").manual_seed(seed)
     v = torch.randn(shape, device="cuda", generator=gen)
     v /= v.norm()
     lam = 0.0
-    p = plan.alloc_raw()
-    w = plan.alloc_native()
     for _ in range(iters):
-        plan.adjoint(v, p)
-        plan.forward_native(p, w)
+        with torch.no_grad():
+            if operator == "matched":
+                w = _p0t(plan, _p0(plan, v))
+            else:
+                p = plan.alloc_raw()
+                plan.adjoint(v, p)
+                w = plan.alloc_native()
+                plan.forward_native(p, w)
         lam = float(torch.dot(w.flatten().double(), v.flatten().double()))
         v = w / w.norm()
     return lam
 
 
 def refine(plan, raw, x0, steps: int = 10, mode: str = "store", lam: float = None,
-           callback=None):
-    """Gradient descent on 0.5 ||A^T x - p||^2 from x0, step 1/lam.
+           callback=None, operator: str = "matched"):
+    """Gradient descent on 0.5 ||k P x - p||^2 from x0 with step 1 / (k^2 lam).
 
-    mode "store": autograd keeps the sinogram-sized residual A^T x - p for the
+    operator "matched": P = P0 = R^T B^T, k = <P0 x0, p> / <P0 x0, P0 x0>;
+    operator "adjoint": P = A^T, k = 1.
+    mode "store": autograd keeps the sinogram-sized residual k P x - p for the
     backward pass; mode "recompute": torch.utils.checkpoint drops it and
-    recomputes A^T x in the backward pass (one extra adjoint per step, one
+    recomputes P x in the backward pass (one more projection per step, one
     sinogram buffer less).  Returns (x, stats).
     """
     from torch.utils.checkpoint import checkpoint
 
+    proj = matched_project if operator == "matched" else project
+    with torch.no_grad():
+        if operator == "matched":
+            px0 = matched_project(x0, plan)
+            k = float(torch.dot(px0.flatten().double(), raw.flatten().double()) /
+                      torch.dot(px0.flatten().double(), px0.flatten().double()))
+            del px0
+        else:
+            k = 1.0
     if lam is None:
-        lam = power_iteration(plan, x0.shape)
-    tau = 1.0 / lam
+        lam = power_iteration(plan, x0.shape, operator=operator)
+    tau = 1.0 / (k * k * lam)
     x = x0.detach().clone()
-    stats = {"lambda_max": lam, "step": tau, "loss": [], "ms_per_step": [], "peak_mb": []}
+    stats = {"operator": operator, "scale": k, "lambda_max": lam, "step": tau, "loss": [],
+             "ms_per_step": [], "peak_mb": []}
 
     def residual_loss(xx):
-        r = project(xx, plan) - raw
+        r = k * proj(xx, plan) - raw
         return 0.5 * (r * r).sum()
 
     for it in range(steps):
@@ -110,7 +165,7 @@ def refine(plan, raw, x0, steps: int = 10, mode: str = "store", lam: float = Non
         torch.cuda.synchronize()
         stats["ms_per_step"].append((time.perf_counter() - t0) * 1e3)
         stats["peak_mb"].append(torch.cuda.max_memory_allocated() / 2 ** 20)
-        stats["loss"].append(float(loss))
+        stats["loss"].append(float(loss.detach()))
         if callback:
             callback(it, x)
     return x, stats

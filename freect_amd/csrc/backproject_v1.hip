@@ -155,10 +155,11 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   float zs[TZ];
 #pragma unroll
   for (int i = 0; i < TZ; ++i) zs[i] = ld_uniform(a.zs + s0 + (i < nz ? i : nz - 1));
-  // Slab offset inside its chunk, in slices (view positions are chunk-relative).
-  const int rel = s0 - chunk * fct_tables::kSliceChunk;
+  // Slice spacing inside the slab (uniform, FP32; used only for skipping).
+  const float zfirst = zs[0], zlast = zs[nz - 1];
+  const float dzs = nz > 1 ? (zlast - zfirst) / (float)(nz - 1) : 1.0f;
+  const float inv_dzs = 1.0f / dzs;
   const float* ztc = a.zt + (size_t)chunk * a.nviews;
-  const int* zpc = a.zpos + (size_t)chunk * a.nviews;
   const int* wrc = reinterpret_cast<const int*>(a.wrange + (size_t)chunk * a.nviews);
   const int* vrp = reinterpret_cast<const int*>(a.slab_vrange + slab);
   const int2 vr = make_int2(ld_uniform(vrp), ld_uniform(vrp + 1));
@@ -217,45 +218,23 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
       const bool chan_ok = active && pfl >= 0.0f && pfl <= (float)(g.ncols - 2);
       const uint32_t pf = chan_ok ? (uint32_t)(int)pfl : 0u;
       const float wp = pidx - pfl;
-      // Over the wave, the z-interval (relative to the view's table z) of
-      // |q| <= 1, and the one where every lane has |q| <= Q, i.e. W = 1 (the
-      // flat part of cuda_kernels.cuh:79-93).  In 24.8 fixed-point native
-      // slices: per view the ranges are then scalar integer arithmetic on the
-      // view's fixed-point slice position zpos.
-      int lo_fx, hi_fx, flo_fx = 1, fhi_fx = 0;
-      {
-        const float A = wave_min(active ? -kbs - lt : BIG) * a.inv_dzn_fx;
-        const float B = wave_max(active ? -kbs + lt : -BIG) * a.inv_dzn_fx;
-        lo_fx = (int)__builtin_amdgcn_readfirstlane((int)floorf(fminf(A, B)));
-        hi_fx = (int)__builtin_amdgcn_readfirstlane((int)ceilf(fmaxf(A, B)));
-        if (MODE == kBpAccumulate) {
-          const float A2 = wave_max(active ? -kbs - FCT_WQ * lt : -BIG);
-          const float B2 = wave_min(active ? -kbs + FCT_WQ * lt : BIG);
-          if (A2 <= B2) {
-            const float C = A2 * a.inv_dzn_fx, D = B2 * a.inv_dzn_fx;
-            flo_fx = (int)__builtin_amdgcn_readfirstlane((int)ceilf(fminf(C, D)));
-            fhi_fx = (int)__builtin_amdgcn_readfirstlane((int)floorf(fmaxf(C, D)));
-          }
-        }
-      }
+      // z-interval of |q| <= 1 relative to the view's table z, over the wave.
+      const float A = wave_min(active ? -kbs - lt : BIG);
+      const float B = wave_max(active ? -kbs + lt : -BIG);
       for (; k < kmax; k += 2) {
         const int v = t + k * H;
         const float ztv = ld_uniform(ztc + v);
-        const int P = ld_uniform(zpc + v);   // view z in 24.8 chunk slices
         // Exact window slice range (native indices).
         const int2 wr = make_int2(ld_uniform(wrc + 2 * v), ld_uniform(wrc + 2 * v + 1));
-        // Candidate slices: the |q| <= 1 interval widened by one slice each
-        // side (fixed-point rounding), cut to the window's exact range.
-        const int ilo = max(max(0, ((P + lo_fx) >> 8) - 1 - rel), wr.x - s0);
-        const int ihi = min(min(nz - 1, ((P + hi_fx + 255) >> 8) + 1 - rel), wr.y - s0);
-        if (ilo > ihi) continue;
-        // Slices flat for every lane, shrunk by 3/256 slice (W is 1 - O(dq^2)
-        // at |q| = Q, so the margin only has to absorb rounding).
-        int flo = TZ, fhi = -1;
-        if (MODE == kBpAccumulate && flo_fx <= fhi_fx) {
-          flo = ((P + flo_fx + 258) >> 8) - rel;
-          fhi = ((P + fhi_fx - 3) >> 8) - rel;
+        float fa = (ztv + A - zfirst) * inv_dzs, fbb = (ztv + B - zfirst) * inv_dzs;
+        if (fa > fbb) {
+          const float tmp = fa;
+          fa = fbb;
+          fbb = tmp;
         }
+        const int ilo = max(max(0, (int)floorf(fa) - 1), wr.x - s0);
+        const int ihi = min(min(nz - 1, (int)ceilf(fbb) + 1), wr.y - s0);
+        if (ilo > ihi) continue;
         ilo_t = min(ilo_t, ilo);
         ihi_t = max(ihi_t, ihi);
         const float row0 = fmaf(kbs - ztv, invLh, g.cr);   // fct_bp_row0
@@ -290,11 +269,7 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
             hi *= __uint_as_float(sc[1]);
           }
           const float val = fmaf(qi - qfl, hi - lo, lo);
-          float W;
-          if (i >= flo && i <= fhi)   // wave-uniform
-            W = sel ? 1.0f : 0.0f;
-          else
-            W = sel ? fct_w_of_row(qi, g) : 0.0f;
+          const float W = sel ? fct_w_of_row(qi, g) : 0.0f;
           num[i] = fmaf(W, val, num[i]);
           den[i] += W;
         }

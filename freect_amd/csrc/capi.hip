@@ -150,7 +150,7 @@ void free_plan_buffers(fct_wfbp_gpu_plan* p) {
                   p->d_slab_vrange, p->d_rebin,  p->d_Hn,        p->d_tw,       p->d_thk_off,
                   p->d_thk_idx,  p->d_thk_w,     p->d_filtered,  p->d_row_scale, p->d_native,
                   p->d_raw_stage, p->d_vol_stage, p->d_gbar, p->d_thkT_off, p->d_thkT_idx,
-                  p->d_thkT_w, p->d_wrange, p->d_zpos};
+                  p->d_thkT_w, p->d_wrange};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
 }
@@ -214,8 +214,6 @@ BpArgs plan_bp_args(const fct_wfbp_gpu_plan* p) {
   a.zt = p->d_zt;
   a.slab_vrange = p->d_slab_vrange;
   a.wrange = p->d_wrange;
-  a.zpos = p->d_zpos;
-  a.inv_dzn_fx = (float)(256.0 / p->dzn);
   a.g = p->g;
   a.scale = p->scale;
   a.nx = p->recon.nx;
@@ -398,17 +396,6 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
     }
   }
 
-  // View table z in native slices from each chunk's first slice, 24.8 fixed
-  // point (the lean kernel's per-view candidate ranges, with a one-slice
-  // margin for this rounding).
-  p->dzn = p->n_native > 1 ? zn[1] - zn[0] : (r.end_pos >= r.start_pos ? p->csw : -p->csw);
-  std::vector<int32_t> zpos((size_t)p->nchunks * p->nviews);
-  for (int c = 0; c < p->nchunks; ++c)
-    for (int v = 0; v < p->nviews; ++v) {
-      const double P = (s.z0 + p->dz * (double)v - zn[(size_t)c * chunk]) / p->dzn * 256.0;
-      zpos[(size_t)c * p->nviews + v] = (int32_t)std::llround(std::max(-1e9, std::min(1e9, P)));
-    }
-
   // Rebinning table (cuda_kernels.cuh:17-36), one per effective-row parity
   // with z FFS; without FFS exactly fct_tables::rebin_entry.
   const int zpar = fct_ffs::has_z(p->ffs.mode) ? 2 : 1;
@@ -490,7 +477,6 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   PLAN_TRY(upload(&p->d_zt, zt, p->stream));
   PLAN_TRY(upload(&p->d_slab_vrange, vr, p->stream));
   PLAN_TRY(upload(&p->d_wrange, wrange, p->stream));
-  PLAN_TRY(upload(&p->d_zpos, zpos, p->stream));
   PLAN_TRY(upload(&p->d_rebin, tab, p->stream));
   PLAN_TRY(upload(&p->d_Hn, Hn, p->stream));
   PLAN_TRY(upload(&p->d_tw, tw, p->stream));
@@ -601,6 +587,33 @@ FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* p, const float* native, floa
   a.plane = p->recon.nx * p->recon.ny;
   a.n_req = p->n_req;
   HIP_TRY(p->ctx, fct::launch_thicken(a, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_rebin(fct_wfbp_gpu_plan* p, const float* raw, float* lines_out) {
+  if (!p || !raw || !lines_out) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  fct::RebinFilterArgs a{};
+  a.raw = raw;
+  a.out = lines_out;
+  a.tab = p->d_rebin;
+  a.Hn = p->d_Hn;
+  a.tw = p->d_tw;
+  a.nviews = p->nviews;
+  a.nrows = p->nrows;
+  a.ncols = p->ncols;
+  a.cpitch = p->cpitch;
+  a.nfft = p->nfft;
+  a.log2n = p->log2n;
+  a.precision = FCT_PRECISION_FP32;
+  a.nlines = p->nviews * p->nrows;
+  a.ncu = p->ncu;
+  a.raw_nrows = p->raw_scan.n_rows;
+  a.raw_ncols = p->raw_scan.n_channels;
+  a.states = p->ffs.S;
+  a.zshift = fct_ffs::has_z(p->ffs.mode) ? 1 : 0;
+  a.rebin_only = 1;
+  HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
   return FCT_OK;
 }
 

@@ -108,7 +108,13 @@ __global__ __launch_bounds__(kRfThreads) void rebin_filter_kernel(RebinFilterArg
       }
     }
     buf0[c] = FctCplx{va, vb};
+    if (a.rebin_only && c < a.cpitch) {   // rebinned rows, FP32, no filter
+      float* out = static_cast<float*>(a.out);
+      out[(size_t)l0 * a.cpitch + c] = va;
+      if (has1) out[(size_t)l1 * a.cpitch + c] = vb;
+    }
   }
+  if (a.rebin_only) return;
   __syncthreads();
   const FctCplx* tw = reinterpret_cast<const FctCplx*>(a.tw);
   FctCplx* X = fft_forward(buf0, buf1, tw, N, a.log2n);
@@ -278,6 +284,19 @@ __global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs
       }
       v[m] = FctCplx{va, vb};
       v[m + 4] = FctCplx{0.0f, 0.0f};
+    }
+    if (a.rebin_only) {   // rebinned rows, FP32, no filter (no LDS exchange: the
+                          // exchange parity of the next pair is unchanged)
+      float* out = static_cast<float*>(a.out);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int c = t + m * T;
+        if (c < a.cpitch) {
+          out[(size_t)l0 * a.cpitch + c] = v[m].x;
+          if (has1) out[(size_t)l1 * a.cpitch + c] = v[m].y;
+        }
+      }
+      continue;
     }
     fft_forward_regs<N, kPingPong, 0>(v, lds, t, tw1, tw2, tw3);
     FctCplx w[8];

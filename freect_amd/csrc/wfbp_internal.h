@@ -39,6 +39,7 @@ struct RebinFilterArgs {
   int32_t raw_nrows, raw_ncols;   // raw detector (differs from nrows / ncols with FFS)
   int32_t states;            // focal states S (1 without FFS)
   int32_t zshift;            // 1 with z FFS: effective row q = 2 r + parity
+  int32_t rebin_only;        // 1: write the rebinned rows (FP32) without filtering
 };
 
 // Everything the backprojection kernel reads.
@@ -56,9 +57,6 @@ struct BpArgs {
   const float* zt;           // [nchunks][nviews] view z relative to the chunk reference
   const int2* slab_vrange;   // [nslabs] views [x, y) a slab can select
   const int2* wrange;        // [nchunks][nviews] native slices [x, y] inside the view window
-  const int32_t* zpos;       // [nchunks][nviews] view z in native slices from the chunk's
-                             // first slice, 24.8 fixed point (rounded)
-  float inv_dzn_fx;          // 256 / native slice spacing (signed), mm -> 24.8 slices
   FctBpConsts g;
   float scale;               // 2 pi / ppr
   int32_t nx, ny, n_native, nviews, H, cpitch;
@@ -156,7 +154,6 @@ struct fct_wfbp_gpu_plan {
   int n_native = 0, n_req = 0, nchunks = 0, tile_z = 16, variant = 0, nslabs = 0;
   int ncu = 0;                      // compute units of the plan's device
   double dgamma = 0, ds = 0, dtheta = 0, dz = 0, csw = 0, acq = 0;
-  double dzn = 0;                   // native slice spacing (signed)
   FctBpConsts g{};
   float scale = 0;
 
@@ -168,7 +165,6 @@ struct fct_wfbp_gpu_plan {
   float* d_zt = nullptr;
   int2* d_slab_vrange = nullptr;
   int2* d_wrange = nullptr;
-  int32_t* d_zpos = nullptr;
   fct::RebinDev* d_rebin = nullptr;
   float* d_Hn = nullptr;
   float2* d_tw = nullptr;

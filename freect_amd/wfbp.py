@@ -98,6 +98,7 @@ SIGNATURES = {
     "fct_wfbp_gpu_phantom_project": (C.c_int, [_VP, _P(ScanGeometryC), C.c_double, C.c_double,
                                                _P(EllipsoidC), C.c_int32, C.c_double,
                                                C.c_uint64, C.c_int32, C.c_int32, _VP, _VP]),
+    "fct_wfbp_gpu_rebin": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_batch_create": (C.c_int, [_VP, C.c_int32, _P(ScanGeometryC),
                                             _P(ReconConfigC), _P(ExecOptionsC), _VP, _P(_VP)]),
     "fct_wfbp_gpu_batch_destroy": (C.c_int, [_VP]),
@@ -331,6 +332,11 @@ class Plan:
     def filter_lines(self, lines_in, lines_out):
         self.ctx.check(load_library().fct_wfbp_gpu_filter(self.h, _ptr(lines_in), _ptr(lines_out)),
                        "filter")
+
+    def rebin_lines(self, raw, lines_out):
+        """Rebinning without the filter: FP32 [view][row][channel_pitch]."""
+        self.ctx.check(load_library().fct_wfbp_gpu_rebin(self.h, _ptr(raw), _ptr(lines_out)),
+                       "rebin")
 
     def rebin_adjoint(self, lines_bar, raw_bar):
         self.ctx.check(load_library().fct_wfbp_gpu_rebin_adjoint(self.h, _ptr(lines_bar),

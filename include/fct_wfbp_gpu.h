@@ -196,6 +196,12 @@ FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* plan, const float* raw, 
 FCT_API int fct_wfbp_gpu_backproject_adjoint(fct_wfbp_gpu_plan* plan, const float* native_bar,
                                              float* filtered_bar);
 FCT_API int fct_wfbp_gpu_filter(fct_wfbp_gpu_plan* plan, const float* lines_in, float* lines_out);
+/* Rebinning alone (rebin_kernel, cuda_kernels.cuh:11-39, without the filter):
+ * raw fan data -> FP32 parallel rows [view][row][channel_pitch].  With
+ * fct_wfbp_gpu_backproject on an FP32 plan it forms the unfiltered chain
+ * B R whose transpose R^T B^T (fct_wfbp_gpu_backproject_adjoint then
+ * fct_wfbp_gpu_rebin_adjoint) is the matched helical projector. */
+FCT_API int fct_wfbp_gpu_rebin(fct_wfbp_gpu_plan* plan, const float* raw, float* lines_out);
 FCT_API int fct_wfbp_gpu_rebin_adjoint(fct_wfbp_gpu_plan* plan, const float* filtered_bar,
                                        float* raw_bar);
 FCT_API int fct_wfbp_gpu_thicken_adjoint(fct_wfbp_gpu_plan* plan, const float* volume_bar,
