@@ -226,9 +226,8 @@ def bench_adjoint(args, torch):
     512^2 x 64 slices, noiseless): WFBP volume x0, then 10 gradient steps on
     0.5 ||k P x - p||^2 with step 1 / (k^2 lambda_max) from power iteration,
     keeping the sinogram-sized residual ("store") or recomputing it in the
-    backward pass ("recompute"); P the matched projector R^T B^T (the
-    hand-written transposed kernels without the filter stage) or the exact
-    transpose A^T of the whole WFBP chain.  HU error of every iterate against
+    backward pass ("recompute"); P the matched projector R^T M^T or the
+    exact transpose A^T of the whole WFBP chain.  HU error of every iterate against
     the noiseless phantom on the native grid."""
     import numpy as np
     from freect_amd import autograd as AG
@@ -260,11 +259,14 @@ def bench_adjoint(args, torch):
         return float(np.sqrt(np.mean(d * d)))
 
     out_modes = {}
+    mask = AG.inside_rays(plan)
+    log("data term on", int(mask.sum()), "of", mask.numel(), "(view, row) rays")
     for op, modes in (("matched", ("store", "recompute")), ("adjoint", ("store",))):
         lam = AG.power_iteration(plan, x0.shape, operator=op)
         for mode in modes:
             hist = []
             x, st = AG.refine(plan, raw, x0, steps=10, mode=mode, lam=lam, operator=op,
+                              mask=mask if op == "matched" else None,
                               callback=lambda it, xx: hist.append(hu_rmse(xx)))
             key = f"{op}/{mode}"
             out_modes[key] = {"hu_rmse_per_step": hist, "loss": st["loss"],
@@ -280,8 +282,9 @@ def bench_adjoint(args, torch):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded ellipsoid phantom, noiseless, generated on the GPU)",
         "config": {"workload": "adjoint: 888x64, 2952 views (3 rotations), 512^2 x 64 slices",
-                   "objective": "0.5 ||k P x - p||^2; matched: P = R^T B^T (unfiltered "
-                                "transposed chain, k least squares at x0); adjoint: P = A^T",
+                   "objective": "0.5 ||m (k P x - p)||^2; matched: P = R^T M^T (voxel-driven "
+                                "R/l-weighted projector, k least squares at x0, m the rays "
+                                "inside the volume's z-range); adjoint: P = A^T (k = 1, m = 1)",
                    "hu_mask": "native slices n/4..3n/4 step 4, r < 0.45 FOV"},
         "hu_rmse_wfbp": hu_rmse(x0),
         "modes": out_modes,

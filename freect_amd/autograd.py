@@ -3,9 +3,11 @@
 A   : raw fan data p -> native volume x (rebin -> filter -> W-weighted
       backprojection, the FP32 plan of wfbp.Plan).
 A^T : its exact transpose (wfbp.Plan.adjoint, HIP kernels in csrc/adjoint.hip).
-P0  : R^T B^T, the same transposed kernels without the filter stage (B the
-      W-weighted backprojection, R the rebinning), a matched helical
-      projector; P0^T = B R is the forward chain without the filter.
+P0  : R^T M^T, a matched helical projector: M^T projects the volume onto the
+      parallel rows through the backprojection's own bilinear footprint with
+      weight R / l over every view that sees the voxel (adjoint.hip, matched
+      mode), R^T is the rebinning transpose; P0^T = M R (rebinning, then the
+      R / l-weighted plain sum of backproject_v1.hip's matched mode).
 
 WFBPFunction.apply(p) = A p with backward g -> A^T g; ProjectFunction.apply(x)
 = A^T x with backward g -> A g; MatchedProjectFunction.apply(x) = P0 x with
@@ -55,20 +57,20 @@ class ProjectFunction(torch.autograd.Function):
 
 
 def _p0(plan, x):
-    """P0 x = R^T B^T x (raw fan data)."""
+    """P0 x = R^T M^T x (raw fan data), M^T the matched voxel-driven projector."""
     lines = plan.alloc_lines()
-    plan.backproject_adjoint(x.contiguous(), lines)   # zero-fills, then scatters
+    plan.matched_project(x.contiguous(), lines)   # zero-fills, then scatters
     p = plan.alloc_raw()
     plan.rebin_adjoint(lines, p)
     return p
 
 
 def _p0t(plan, g):
-    """P0^T g = B R g (native volume)."""
+    """P0^T g = M R g (native volume), M the R / l-weighted plain sum."""
     lines = plan.alloc_lines()
     plan.rebin_lines(g.contiguous(), lines)
     x = plan.alloc_native()
-    plan.backproject(lines, None, x)
+    plan.matched_backproject(lines, x)
     return x
 
 
@@ -117,12 +119,38 @@ def power_iteration(plan, shape, iters: int = 12, seed: int = 0,
     return lam
 
 
+def inside_rays(plan, margin: float = 1.0):
+    """[view][row][1] mask of the rays whose whole path through the
+    reconstruction FOV lies inside the volume's z-range (shrunk by `margin`
+    mm): only those are explained by the reconstructed slab.  The ray of
+    detector row r leaves the source at the view's table z and climbs
+    (r - cr) drow per Dsd of in-plane travel; inside the FOV the in-plane
+    distance from the source is within R -+ FOV/2."""
+    sc, rc = plan.scan, plan.recon
+    if sc.ffs_mode:
+        raise ValueError("inside_rays: FFS scans are not supported")
+    v = torch.arange(sc.n_views, device="cuda", dtype=torch.float64)
+    zsrc = sc.z0 + v * sc.feed / sc.views_per_rotation
+    w = (torch.arange(sc.n_rows, device="cuda", dtype=torch.float64) - sc.central_row) * \
+        sc.row_spacing
+    half = 0.5 * rc.recon_fov * 2 ** 0.5
+    lo_l, hi_l = max(sc.r - half, 0.0), sc.r + half
+    za = zsrc[:, None] + w[None, :] * (lo_l / sc.dsd)
+    zb = zsrc[:, None] + w[None, :] * (hi_l / sc.dsd)
+    zmin = min(rc.start_pos, rc.end_pos) + margin
+    zmax = max(rc.start_pos, rc.end_pos) - margin
+    ok = (torch.minimum(za, zb) >= zmin) & (torch.maximum(za, zb) <= zmax)
+    return ok[..., None].float()
+
+
 def refine(plan, raw, x0, steps: int = 10, mode: str = "store", lam: float = None,
-           callback=None, operator: str = "matched"):
+           callback=None, operator: str = "matched", mask=None):
     """Gradient descent on 0.5 ||k P x - p||^2 from x0 with step 1 / (k^2 lam).
 
-    operator "matched": P = P0 = R^T B^T, k = <P0 x0, p> / <P0 x0, P0 x0>;
+    operator "matched": P = P0 = R^T M^T, k = <P0 x0, p> / <P0 x0, P0 x0>;
     operator "adjoint": P = A^T, k = 1.
+    mask (broadcastable to the raw data, e.g. inside_rays(plan)) restricts
+    the data term to rays the volume can explain: 0.5 ||m (k P x - p)||^2.
     mode "store": autograd keeps the sinogram-sized residual k P x - p for the
     backward pass; mode "recompute": torch.utils.checkpoint drops it and
     recomputes P x in the backward pass (one more projection per step, one
@@ -131,10 +159,11 @@ def refine(plan, raw, x0, steps: int = 10, mode: str = "store", lam: float = Non
     from torch.utils.checkpoint import checkpoint
 
     proj = matched_project if operator == "matched" else project
+    m = mask if mask is not None else torch.ones((), device=raw.device)
     with torch.no_grad():
         if operator == "matched":
-            px0 = matched_project(x0, plan)
-            k = float(torch.dot(px0.flatten().double(), raw.flatten().double()) /
+            px0 = matched_project(x0, plan) * m
+            k = float(torch.dot(px0.flatten().double(), (raw * m).flatten().double()) /
                       torch.dot(px0.flatten().double(), px0.flatten().double()))
             del px0
         else:
@@ -147,7 +176,7 @@ def refine(plan, raw, x0, steps: int = 10, mode: str = "store", lam: float = Non
              "ms_per_step": [], "peak_mb": []}
 
     def residual_loss(xx):
-        r = k * proj(xx, plan) - raw
+        r = (k * proj(xx, plan) - raw) * m
         return 0.5 * (r * r).sum()
 
     for it in range(steps):

@@ -19,7 +19,11 @@
 namespace fct {
 namespace {
 
-template <int TZ, bool CENTERED>
+// MATCHED: the transpose of backproject_v1.hip's matched mode instead --
+// every view that sees the voxel, weight scale * R / l, no W normalisation
+// and no view window: a voxel-driven helical projector of the native volume
+// onto the parallel rows (with the rebinning transpose, onto the fan data).
+template <int TZ, bool CENTERED, bool MATCHED>
 __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
   const BpArgs& b = a.bp;
   const int lane = threadIdx.x;
@@ -60,11 +64,13 @@ __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
     const float kb = g.kappa * fct_asinf(p * g.invR);
     const float pidx_e = fmaf(p, g.inv_ds, g.cc), pidx_o = fmaf(-p, g.inv_ds, g.cc);
     const float invL_e = g.hr / ((sq - f) * g.tanc), invL_o = g.hr / ((sq + f) * g.tanc);
+    const float fR_e = g.tanc / ((sq - f) * g.tanc * g.invR);   // R / l (matched mode)
+    const float fR_o = g.tanc / ((sq + f) * g.tanc * g.invR);
     float den[TZ];
 #pragma unroll
     for (int i = 0; i < TZ; ++i) den[i] = 0.0f;
     // Pass 0: weight sums; pass 1: scatter.
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = MATCHED ? 1 : 0; pass < 2; ++pass) {
       float rden[TZ];
       if (pass == 1) {
 #pragma unroll
@@ -84,7 +90,7 @@ __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
         float* vb = a.gbar + (size_t)v * vstride + pf;
 #pragma unroll
         for (int i = 0; i < TZ; ++i) {
-          if (i >= nz || !fct_bp_in_window(ztv, zs[i], g)) continue;
+          if (i >= nz || (!MATCHED && !fct_bp_in_window(ztv, zs[i], g))) continue;
           const float qi = fmaf(zs[i], invL, row0);   // fct_bp_row
           const float qfl = floorf(qi);
           bool sel;
@@ -94,6 +100,17 @@ __global__ __launch_bounds__(64) void bp_adjoint_kernel(BpAdjArgs a) {
             sel = chan_ok && !(fabsf(qi - g.cr) > g.hr) && qfl >= 0.0f &&
                   qfl <= (float)(g.nrows - 2);
           if (!sel) continue;
+          if (MATCHED) {
+            const float c = xb[i] * (odd ? fR_o : fR_e);
+            const float wq = qi - qfl;
+            float* r0 = vb + (size_t)(int)qfl * b.cpitch;
+            float* r1 = r0 + b.cpitch;
+            unsafeAtomicAdd(r0, c * (1.0f - wq) * (1.0f - wp));
+            unsafeAtomicAdd(r0 + 1, c * (1.0f - wq) * wp);
+            unsafeAtomicAdd(r1, c * wq * (1.0f - wp));
+            unsafeAtomicAdd(r1 + 1, c * wq * wp);
+            continue;
+          }
           const float W = fct_w_of_row(qi, g);
           if (pass == 0) {
             den[i] += W;
@@ -145,23 +162,31 @@ __global__ __launch_bounds__(256) void fill_kernel(float* p, size_t n, float v) 
 }
 
 template <int TZ>
-hipError_t launch_tz(const BpAdjArgs& a, hipStream_t s) {
+hipError_t launch_tz(const BpAdjArgs& a, hipStream_t s, bool matched) {
   const int blocks = a.bp.tiles_x * a.bp.tiles_y * a.bp.nslabs;
-  if (a.bp.g.cr == a.bp.g.hr)
-    hipLaunchKernelGGL((bp_adjoint_kernel<TZ, true>), dim3(blocks), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL((bp_adjoint_kernel<TZ, false>), dim3(blocks), dim3(64), 0, s, a);
+  const bool c = a.bp.g.cr == a.bp.g.hr;
+  if (matched) {
+    if (c)
+      hipLaunchKernelGGL((bp_adjoint_kernel<TZ, true, true>), dim3(blocks), dim3(64), 0, s, a);
+    else
+      hipLaunchKernelGGL((bp_adjoint_kernel<TZ, false, true>), dim3(blocks), dim3(64), 0, s, a);
+  } else if (c) {
+    hipLaunchKernelGGL((bp_adjoint_kernel<TZ, true, false>), dim3(blocks), dim3(64), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((bp_adjoint_kernel<TZ, false, false>), dim3(blocks), dim3(64), 0, s, a);
+  }
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_backproject_adjoint(const BpAdjArgs& a, int tile_z, hipStream_t s) {
+hipError_t launch_backproject_adjoint(const BpAdjArgs& a, int tile_z, hipStream_t s,
+                                      bool matched) {
   switch (tile_z) {
-    case 4: return launch_tz<4>(a, s);
-    case 8: return launch_tz<8>(a, s);
-    case 16: return launch_tz<16>(a, s);
-    case 32: return launch_tz<32>(a, s);
+    case 4: return launch_tz<4>(a, s, matched);
+    case 8: return launch_tz<8>(a, s, matched);
+    case 16: return launch_tz<16>(a, s, matched);
+    case 32: return launch_tz<32>(a, s, matched);
   }
   return hipErrorInvalidValue;
 }

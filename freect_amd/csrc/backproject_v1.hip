@@ -146,7 +146,7 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
     if (inb)
       for (int i = 0; i < nz; ++i) {
         const size_t o = ((size_t)(s0 + i) * a.ny + iy) * a.nx + ix;
-        if (MODE == kBpAccumulate) a.native[o] = 0.0f;
+        if (MODE == kBpAccumulate || MODE == kBpMatched) a.native[o] = 0.0f;
         if (MODE == kBpCount) a.counts[o] = 0;
       }
     return;
@@ -218,6 +218,9 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
       const bool chan_ok = active && pfl >= 0.0f && pfl <= (float)(g.ncols - 2);
       const uint32_t pf = chan_ok ? (uint32_t)(int)pfl : 0u;
       const float wp = pidx - pfl;
+      // Matched mode: R / l, the voxel-driven footprint's depth correction
+      // (a row spans drow l / Dsd of z at distance l from the source).
+      const float fR = MODE == kBpMatched ? g.tanc / (lt * g.invR) : 0.0f;
       // z-interval of |q| <= 1 relative to the view's table z, over the wave.
       const float A = wave_min(active ? -kbs - lt : BIG);
       const float B = wave_max(active ? -kbs + lt : -BIG);
@@ -232,8 +235,11 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
           fa = fbb;
           fbb = tmp;
         }
-        const int ilo = max(max(0, (int)floorf(fa) - 1), wr.x - s0);
-        const int ihi = min(min(nz - 1, (int)ceilf(fbb) + 1), wr.y - s0);
+        // (The matched projector pair uses every view that sees the voxel:
+        // no view window.)
+        const int ilo = max(max(0, (int)floorf(fa) - 1), MODE == kBpMatched ? 0 : wr.x - s0);
+        const int ihi = min(min(nz - 1, (int)ceilf(fbb) + 1),
+                            MODE == kBpMatched ? nz - 1 : wr.y - s0);
         if (ilo > ihi) continue;
         ilo_t = min(ilo_t, ilo);
         ihi_t = max(ihi_t, ihi);
@@ -269,9 +275,13 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
             hi *= __uint_as_float(sc[1]);
           }
           const float val = fmaf(qi - qfl, hi - lo, lo);
-          const float W = sel ? fct_w_of_row(qi, g) : 0.0f;
-          num[i] = fmaf(W, val, num[i]);
-          den[i] += W;
+          if (MODE == kBpMatched) {   // plain sum over the views, weight R / l
+            num[i] = fmaf(sel ? fR : 0.0f, val, num[i]);
+          } else {
+            const float W = sel ? fct_w_of_row(qi, g) : 0.0f;
+            num[i] = fmaf(W, val, num[i]);
+            den[i] += W;
+          }
         }
       }
     }
@@ -280,10 +290,13 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
       for (int i = 0; i < TZ; ++i)
         if (i >= ilo_t && i <= ihi_t)
           acc[i] = fmaf(num[i], __builtin_amdgcn_rcpf(fmaxf(den[i], 1e-30f)), acc[i]);
+    } else if (MODE == kBpMatched) {
+#pragma unroll
+      for (int i = 0; i < TZ; ++i) acc[i] += num[i];
     }
   }
   if (!inb) return;
-  if (MODE == kBpAccumulate) {
+  if (MODE == kBpAccumulate || MODE == kBpMatched) {
 #pragma unroll
     for (int i = 0; i < TZ; ++i)
       if (i < nz) a.native[((size_t)(s0 + i) * a.ny + iy) * a.nx + ix] = acc[i] * a.scale;
@@ -342,6 +355,8 @@ hipError_t launch3(const BpArgs& a, int mode, hipStream_t s) {
     hipLaunchKernelGGL((bp_v1_kernel<TZ, PREC, kBpAccumulate, C>), dim3(blocks), dim3(64), 0, s, a);
   else if (mode == kBpCount)
     hipLaunchKernelGGL((bp_v1_kernel<TZ, PREC, kBpCount, C>), dim3(blocks), dim3(64), 0, s, a);
+  else if (mode == kBpMatched && PREC == FCT_PRECISION_FP32)
+    hipLaunchKernelGGL((bp_v1_kernel<TZ, PREC, kBpMatched, C>), dim3(blocks), dim3(64), 0, s, a);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -657,6 +657,32 @@ FCT_API int fct_wfbp_gpu_backproject_adjoint(fct_wfbp_gpu_plan* p, const float* 
   return FCT_OK;
 }
 
+FCT_API int fct_wfbp_gpu_matched_project(fct_wfbp_gpu_plan* p, const float* native,
+                                         float* lines) {
+  if (!p || !native || !lines) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  HIP_TRY(p->ctx, fct::launch_fill(lines, (size_t)p->nviews * p->nrows * p->cpitch, 0.0f,
+                                   p->stream));
+  fct::BpAdjArgs a{};
+  a.bp = bp_args(p);
+  a.xbar = native;
+  a.gbar = lines;
+  HIP_TRY(p->ctx, fct::launch_backproject_adjoint(a, p->tile_z, p->stream, true));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_matched_backproject(fct_wfbp_gpu_plan* p, const float* lines,
+                                             float* native) {
+  if (!p || !lines || !native) return FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  fct::BpArgs a = bp_args(p);
+  a.filt = lines;
+  a.native = native;
+  HIP_TRY(p->ctx, fct::launch_backproject(a, p->tile_z, FCT_BP_LEAN, FCT_PRECISION_FP32,
+                                          fct::kBpMatched, p->stream));
+  return FCT_OK;
+}
+
 FCT_API int fct_wfbp_gpu_rebin_adjoint(fct_wfbp_gpu_plan* p, const float* filtered_bar,
                                        float* raw_bar) {
   if (!p || !filtered_bar || !raw_bar) return FCT_ERR_INVALID_ARGUMENT;

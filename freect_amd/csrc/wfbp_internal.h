@@ -110,7 +110,8 @@ struct RebinAdjArgs {
 };
 
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s);
-hipError_t launch_backproject_adjoint(const BpAdjArgs& a, int tile_z, hipStream_t s);
+hipError_t launch_backproject_adjoint(const BpAdjArgs& a, int tile_z, hipStream_t s,
+                                      bool matched = false);
 hipError_t launch_rebin_adjoint(const RebinAdjArgs& a, hipStream_t s);
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_backproject(const BpArgs& a, int tile_z, int variant, int precision, int mode,
@@ -123,7 +124,10 @@ hipError_t backproject_batch_waves_per_cu(int tile_z, int precision, int* waves)
 hipError_t launch_thicken(const ThickenArgs& a, hipStream_t s);
 hipError_t launch_phantom(const PhantomArgs& a, hipStream_t s);
 
-enum BpMode { kBpAccumulate = 0, kBpCount = 1, kBpSelect = 2 };
+// kBpMatched: the plain R / l-weighted sum over every view that sees the
+// voxel (no view window, no W normalisation) -- the transpose of the matched
+// projector of adjoint.hip's matched mode.
+enum BpMode { kBpAccumulate = 0, kBpCount = 1, kBpSelect = 2, kBpMatched = 3 };
 
 }  // namespace fct
 

@@ -99,6 +99,8 @@ SIGNATURES = {
                                                _P(EllipsoidC), C.c_int32, C.c_double,
                                                C.c_uint64, C.c_int32, C.c_int32, _VP, _VP]),
     "fct_wfbp_gpu_rebin": (C.c_int, [_VP, _VP, _VP]),
+    "fct_wfbp_gpu_matched_project": (C.c_int, [_VP, _VP, _VP]),
+    "fct_wfbp_gpu_matched_backproject": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_batch_create": (C.c_int, [_VP, C.c_int32, _P(ScanGeometryC),
                                             _P(ReconConfigC), _P(ExecOptionsC), _VP, _P(_VP)]),
     "fct_wfbp_gpu_batch_destroy": (C.c_int, [_VP]),
@@ -337,6 +339,15 @@ class Plan:
         """Rebinning without the filter: FP32 [view][row][channel_pitch]."""
         self.ctx.check(load_library().fct_wfbp_gpu_rebin(self.h, _ptr(raw), _ptr(lines_out)),
                        "rebin")
+
+    def matched_project(self, native, lines):
+        self.ctx.check(load_library().fct_wfbp_gpu_matched_project(self.h, _ptr(native),
+                                                                  _ptr(lines)), "matched_project")
+
+    def matched_backproject(self, lines, native):
+        self.ctx.check(load_library().fct_wfbp_gpu_matched_backproject(self.h, _ptr(lines),
+                                                                      _ptr(native)),
+                       "matched_backproject")
 
     def rebin_adjoint(self, lines_bar, raw_bar):
         self.ctx.check(load_library().fct_wfbp_gpu_rebin_adjoint(self.h, _ptr(lines_bar),

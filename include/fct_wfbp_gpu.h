@@ -202,6 +202,19 @@ FCT_API int fct_wfbp_gpu_filter(fct_wfbp_gpu_plan* plan, const float* lines_in, 
  * B R whose transpose R^T B^T (fct_wfbp_gpu_backproject_adjoint then
  * fct_wfbp_gpu_rebin_adjoint) is the matched helical projector. */
 FCT_API int fct_wfbp_gpu_rebin(fct_wfbp_gpu_plan* plan, const float* raw, float* lines_out);
+/* Matched helical projector pair (no reference counterpart): the
+ * voxel-driven projection of a native volume onto FP32 parallel rows, every
+ * view that sees a voxel contributing its bilinear footprint with weight
+ * (2 pi / ppr) R / l (l the source-voxel distance in the plane, correcting
+ * the footprint's depth), and its exact transpose, the plain R / l-weighted
+ * sum over the same samples (the W-weighted backprojection without the
+ * weights' normalisation or the view window).  With fct_wfbp_gpu_rebin_adjoint
+ * / fct_wfbp_gpu_rebin this projects onto the fan data; up to one scale it
+ * approximates line integrals.  lines is zero-filled first. */
+FCT_API int fct_wfbp_gpu_matched_project(fct_wfbp_gpu_plan* plan, const float* native,
+                                         float* lines);
+FCT_API int fct_wfbp_gpu_matched_backproject(fct_wfbp_gpu_plan* plan, const float* lines,
+                                             float* native);
 FCT_API int fct_wfbp_gpu_rebin_adjoint(fct_wfbp_gpu_plan* plan, const float* filtered_bar,
                                        float* raw_bar);
 FCT_API int fct_wfbp_gpu_thicken_adjoint(fct_wfbp_gpu_plan* plan, const float* volume_bar,
