@@ -232,8 +232,8 @@ def test_matched_projector_dot_products_and_refinement(ctx):
     rel = float((k * px - ref).norm() / ref.norm())
     say("matched projector: scale %.5g, relative misfit %.4f on %d rays" % (k, rel, int(m.sum())))
     # Coarse here (12.5 mm voxels against 5 mm rows and ~5 mm channels): the
-    # voxel-driven footprint leaves a misfit of ~0.25; a wrong weighting (no
-    # R / l, or the W-normalised backprojection's transpose) leaves > 0.45.
+    # voxel-driven footprint leaves a misfit of ~0.25; the transpose of the
+    # W-normalised backprojection (no R / l, view window) measured 0.49.
     assert rel < 0.35
     x, st = AG.refine(p, raw, x0, steps=4, operator="matched", mask=m)
     say("refine losses", st["loss"])
