@@ -469,11 +469,15 @@ def main():
 
     bpv = BYTES_PER_VV[args.precision]
     achieved = n_vv * bpv / (chosen["bp_ms"] * 1e-3)
+    # HBM bytes per launch of the timed kernel, from the calibrated
+    # FETCH_SIZE / WRITE_SIZE passes committed under profiles/.
     traffic = None
     tp = os.path.join(ROOT, "profiles", "backproject_full_traffic.json")
+    tkey = args.precision if (args.path == "batch" or args.precision == "fp32") \
+        else args.precision + "_plan"
     if os.path.exists(tp):
         try:
-            traffic = json.load(open(tp)).get(args.precision)
+            traffic = json.load(open(tp)).get(tkey, {}).get("traffic_bytes")
         except Exception:
             traffic = None
     out = {
