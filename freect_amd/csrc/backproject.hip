@@ -135,8 +135,8 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
       const bool chan_ok = active && pfl >= 0.0f && pfl <= (float)(g.ncols - 2);
       const int pf = chan_ok ? (int)pfl : 0;
       const float wp = pidx - pfl;
-      const T* vbase = filt + (size_t)v * vstride + pf;
-      const float* rsv = PREC != FCT_PRECISION_FP32 ? a.row_scale + (size_t)v * g.nrows : nullptr;
+      const T* vbase = filt + (size_t)v * vstride + pf;   // FP32 rows
+      const T* qbase = filt + ((size_t)v * vstride + pf) * 4;   // FP16 / BF16 quads
 #pragma unroll
       for (int i = 0; i < TZ; ++i) {
         if (i < nz && fct_bp_in_window(ztv, zs[i], g)) {
@@ -152,14 +152,23 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
           } else if (sel) {
             const int qf = (int)qfl;
             const float wq = qi - qfl;
-            const T* r0 = vbase + (size_t)qf * a.cpitch;
-            const T* r1 = r0 + a.cpitch;
-            float lo = fmaf(wp, E::f(r0[1]) - E::f(r0[0]), E::f(r0[0]));
-            float hi = fmaf(wp, E::f(r1[1]) - E::f(r1[0]), E::f(r1[0]));
-            if (PREC != FCT_PRECISION_FP32) {
-              lo *= rsv[qf];
-              hi *= rsv[qf + 1];
+            float y00, y01, y10, y11;
+            if (PREC == FCT_PRECISION_FP32) {
+              const T* r0 = vbase + (size_t)qf * a.cpitch;
+              const T* r1 = r0 + a.cpitch;
+              y00 = E::f(r0[0]);
+              y01 = E::f(r0[1]);
+              y10 = E::f(r1[0]);
+              y11 = E::f(r1[1]);
+            } else {   // quad entry (qf, pf)
+              const T* qd = qbase + (size_t)qf * a.cpitch * 4;
+              y00 = E::f(qd[0]);
+              y01 = E::f(qd[1]);
+              y10 = E::f(qd[2]);
+              y11 = E::f(qd[3]);
             }
+            const float lo = fmaf(wp, y01 - y00, y00);
+            const float hi = fmaf(wp, y11 - y10, y10);
             const float val = fmaf(wq, hi - lo, lo);
             const float W = fct_w_of_row(qi, g);
             num[i] = fmaf(W, val, num[i]);
@@ -176,9 +185,11 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
   }
   if (!inb) return;
   if (MODE == kBpAccumulate) {
+    // FP16 / BF16 projections carry one storage scale per scan.
+    const float out_scale = PREC == FCT_PRECISION_FP32 ? a.scale : a.scale * a.filt_scale[0];
 #pragma unroll
     for (int i = 0; i < TZ; ++i)
-      if (i < nz) a.native[((size_t)(s0 + i) * a.ny + iy) * a.nx + ix] = acc[i] * a.scale;
+      if (i < nz) a.native[((size_t)(s0 + i) * a.ny + iy) * a.nx + ix] = acc[i] * out_scale;
   } else if (MODE == kBpCount) {
     unsigned long long tot = 0;
 #pragma unroll

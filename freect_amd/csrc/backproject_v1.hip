@@ -2,21 +2,28 @@
 //
 // Same arithmetic and selection as backproject.hip (cuda_kernels.cuh:198-287
 // restated with the shared FP32 code of wfbp_math.h), organised to spend few
-// VALU instructions per selected (voxel, view) sample -- the kernel is
-// VALU-issue-bound:
+// instructions per selected (voxel, view) sample -- the kernel is bound by
+// instruction issue (VALU and vector-memory), not by HBM:
 //   * per (theta~, view parity) each lane's z-interval of |q| <= 1 is
 //     [zt - kb - lt, zt - kb + lt]; a DPP wave reduction of its ends gives,
 //     per view, the slices any lane of the wave can select, intersected with
 //     the view window's exact slice range (a host table built with the same
 //     FP32 test); other slices are skipped wave-uniformly and the exact
 //     per-sample selection still runs, so selection is unchanged;
+//   * the views of one theta~ and parity move monotonically in z, so the
+//     view loop ends at the first view whose slice range lies past the slab;
+//   * when a view covers the whole slab (the common case) all slices are
+//     processed without a per-slice branch, so the taps of every slice are in
+//     flight at once;
 //   * the detector row position is one fma per slice, and when the central
 //     row is (nrows-1)/2 the cone test |q| <= 1 is implied exactly by the row
 //     test, one unsigned compare;
-//   * taps are two 8-byte buffer loads with 32-bit byte offsets (the second
-//     row through the scalar offset); lanes that select nothing read whatever
-//     the range-checked buffer returns and get weight 0 instead of a branch;
-//   * FP16 taps are interpolated across channels with v_dot2_f32_f16;
+//   * taps: FP32 rows are two 8-byte buffer loads (the second row through the
+//     scalar offset); FP16 / BF16 projections are stored as 2 x 2 quads, one
+//     8-byte load per sample, interpolated across channels with
+//     v_dot2_f32_f16 (FP16), and scaled once per scan at the end; lanes that
+//     select nothing read whatever the range-checked buffer returns and get
+//     weight 0 instead of a branch;
 //   * the per-theta~ mean uses v_rcp.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -70,65 +77,136 @@ __device__ __forceinline__ T ld_uniform(const T* p) {
   return *(const __attribute__((address_space(4))) T*)p;
 }
 
-// Per (lane, view) interpolation state across the two channels pf, pf+1.
+// Per (lane, view) interpolation across the channels pf, pf + 1: fetch()
+// issues the loads of rows iq and iq + 1 from the byte offset of (view, iq,
+// pf), rows() returns their channel-interpolated values.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 template <int PREC>
 struct ChanInterp;
 
 template <>
-struct ChanInterp<FCT_PRECISION_FP32> {
+struct ChanInterp<FCT_PRECISION_FP32> {   // plain rows: two 8-byte loads
+  static constexpr uint32_t kEntry = 4;     // bytes per (row, channel)
+  struct Raw {
+    u32x2 a, b;
+  };
   float wp;
   uint32_t base;   // byte offset of (view, row 0, pf)
   __device__ void init(uint32_t view_bytes, uint32_t pf, float w) {
     wp = w;
-    base = view_bytes + pf * 4u;
+    base = view_bytes + pf * kEntry;
   }
-  __device__ float row(__amdgpu_buffer_rsrc_t r, uint32_t off, int soff) const {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, soff, 0);
-    const float a = __uint_as_float(v[0]), b = __uint_as_float(v[1]);
-    return fmaf(wp, b - a, a);
+  __device__ static Raw fetch(__amdgpu_buffer_rsrc_t r, uint32_t off, int rowB) {
+    return Raw{__builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0),
+               __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, rowB, 0)};
+  }
+  __device__ float2 rows(const Raw& t) const {
+    const uint32_t a0 = t.a.x, a1 = t.a.y, b0 = t.b.x, b1 = t.b.y;
+    const float fa0 = __uint_as_float(a0), fa1 = __uint_as_float(a1);
+    const float fb0 = __uint_as_float(b0), fb1 = __uint_as_float(b1);
+    return make_float2(fmaf(wp, fa1 - fa0, fa0), fmaf(wp, fb1 - fb0, fb0));
   }
 };
 
 template <>
-struct ChanInterp<FCT_PRECISION_FP16> {
+struct ChanInterp<FCT_PRECISION_FP16> {   // quad entries: one 8-byte load
+  static constexpr uint32_t kEntry = 8;
+  typedef u32x2 Raw;
   half2_t w2;      // (1 - wp, wp)
-  uint32_t base;   // 4-byte aligned byte offset of (view, row 0, pf & ~1)
-  uint32_t shift;  // 0 or 2: byte position of channel pf inside the loaded pair
+  uint32_t base;
   __device__ void init(uint32_t view_bytes, uint32_t pf, float w) {
     w2 = half2_t{(_Float16)(1.0f - w), (_Float16)w};
-    base = view_bytes + (pf & ~1u) * 2u;
-    shift = (pf & 1u) * 2u;
+    base = view_bytes + pf * kEntry;
   }
-  __device__ float row(__amdgpu_buffer_rsrc_t r, uint32_t off, int soff) const {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, soff, 0);
-    const uint32_t pair = __builtin_amdgcn_alignbyte(v[1], v[0], shift);
-    const half2_t h = __builtin_bit_cast(half2_t, pair);
-    return __builtin_amdgcn_fdot2(h, w2, 0.0f, false);
+  __device__ static Raw fetch(__amdgpu_buffer_rsrc_t r, uint32_t off, int) {
+    return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+  }
+  __device__ float2 rows(const Raw& t) const {
+    // Through named scalars: hipcc (ROCm 7.2) folds __builtin_bit_cast of a
+    // vector element such as t[1] to element 0.
+    const uint32_t r0 = t.x, r1 = t.y;
+    return make_float2(__builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, r0), w2, 0.0f, false),
+                       __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, r1), w2, 0.0f, false));
   }
 };
 
 template <>
-struct ChanInterp<FCT_PRECISION_BF16> {
+struct ChanInterp<FCT_PRECISION_BF16> {   // quad entries: one 8-byte load
+  static constexpr uint32_t kEntry = 8;
+  typedef u32x2 Raw;
   float wp;
   uint32_t base;
-  uint32_t shift;
   __device__ void init(uint32_t view_bytes, uint32_t pf, float w) {
     wp = w;
-    base = view_bytes + (pf & ~1u) * 2u;
-    shift = (pf & 1u) * 2u;
+    base = view_bytes + pf * kEntry;
   }
-  __device__ float row(__amdgpu_buffer_rsrc_t r, uint32_t off, int soff) const {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, soff, 0);
-    const uint32_t pair = __builtin_amdgcn_alignbyte(v[1], v[0], shift);
-    const float a = __uint_as_float(pair << 16), b = __uint_as_float(pair & 0xFFFF0000u);
-    return fmaf(wp, b - a, a);
+  __device__ static Raw fetch(__amdgpu_buffer_rsrc_t r, uint32_t off, int) {
+    return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+  }
+  __device__ float2 rows(const Raw& t) const {
+    const uint32_t r0 = t.x, r1 = t.y;
+    const float a0 = __uint_as_float(r0 << 16), a1 = __uint_as_float(r0 & 0xFFFF0000u);
+    const float b0 = __uint_as_float(r1 << 16), b1 = __uint_as_float(r1 & 0xFFFF0000u);
+    return make_float2(fmaf(wp, a1 - a0, a0), fmaf(wp, b1 - b0, b0));
   }
 };
+
+// One (voxel, view, slice) sample in two steps: bp_fetch computes the row
+// position and the exact selection and issues the tap loads; bp_add
+// interpolates and accumulates into (num, den) or cnt.  Branch-free: lanes
+// that select nothing read whatever the range-checked buffer returns at their
+// offset and get weight 0.
+template <int PREC>
+struct BpTap {
+  float qi, qfl;
+  bool sel;
+  typename ChanInterp<PREC>::Raw raw;
+};
+
+template <int PREC, int MODE, bool CENTERED>
+__device__ __forceinline__ BpTap<PREC> bp_fetch(float zsi, float invLh, float row0,
+                                                const ChanInterp<PREC>& ci,
+                                                __amdgpu_buffer_rsrc_t rf, uint32_t rowB,
+                                                bool chan_ok, const FctBpConsts& g) {
+  BpTap<PREC> s;
+  s.qi = fmaf(zsi, invLh, row0);   // fct_bp_row
+  s.qfl = floorf(s.qi);
+  const int iq = (int)s.qfl;
+  if (CENTERED)   // |q| <= 1 is implied by the row test when cr = hr
+    s.sel = chan_ok && (uint32_t)iq <= (uint32_t)(g.nrows - 2);
+  else
+    s.sel = chan_ok && !(fabsf(s.qi - g.cr) > g.hr) && s.qfl >= 0.0f &&
+            s.qfl <= (float)(g.nrows - 2);
+  // 24-bit multiply: exact for selected rows (< 2^24).
+  if (MODE != kBpCount)
+    s.raw = ChanInterp<PREC>::fetch(rf, ci.base + __umul24((uint32_t)iq, rowB), (int)rowB);
+  return s;
+}
+
+template <int PREC, int MODE>
+__device__ __forceinline__ void bp_add(const BpTap<PREC>& s, const ChanInterp<PREC>& ci, float fR,
+                                       const FctBpConsts& g, float& num, float& den, int& cnt) {
+  if (MODE == kBpCount) {
+    cnt += s.sel ? 1 : 0;
+    return;
+  }
+  const float2 lh = ci.rows(s.raw);
+  const float val = fmaf(s.qi - s.qfl, lh.y - lh.x, lh.x);
+  if (MODE == kBpMatched) {   // plain sum over the views, weight R / l
+    num = fmaf(s.sel ? fR : 0.0f, val, num);
+  } else {
+    const float w = fct_w_of_row(s.qi, g);
+    const float W = s.sel ? w : 0.0f;
+    num = fmaf(W, val, num);
+    den += W;
+  }
+}
 
 // One wave reconstructs tile `lin` (slab-major order) of the job described by a.
 template <int TZ, int PREC, int MODE, bool CENTERED>
 __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
-  constexpr uint32_t kElem = PREC == FCT_PRECISION_FP32 ? 4u : 2u;
+  constexpr uint32_t kEntry = ChanInterp<PREC>::kEntry;
   const int lane = threadIdx.x;
   const int tiles = a.tiles_x * a.tiles_y;
   const int slab = lin / tiles;
@@ -159,24 +237,28 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   const float zfirst = zs[0], zlast = zs[nz - 1];
   const float dzs = nz > 1 ? (zlast - zfirst) / (float)(nz - 1) : 1.0f;
   const float inv_dzs = 1.0f / dzs;
+  // Along k (views t + k H of one theta~ and parity) the view z moves
+  // monotonically, and with it every bound of a view's slice range [ilo, ihi]
+  // below, all in the same direction: up (in slice index) when the table feed
+  // and the slice step have the same sign.  A view whose range lies entirely
+  // past the slab in that direction is followed only by such views.
+  const bool up = (g.kappa > 0.0f) == (dzs > 0.0f);
+  const bool can_stop = nz > 1;
   const float* ztc = a.zt + (size_t)chunk * a.nviews;
   const int* wrc = reinterpret_cast<const int*>(a.wrange + (size_t)chunk * a.nviews);
   const int* vrp = reinterpret_cast<const int*>(a.slab_vrange + slab);
   const int2 vr = make_int2(ld_uniform(vrp), ld_uniform(vrp + 1));
   const int H = a.H;
-  const uint32_t vstride = (uint32_t)g.nrows * (uint32_t)a.cpitch;
-  const uint32_t rowB = (uint32_t)a.cpitch * kElem;   // bytes per filtered row
+  const uint32_t vstride = (uint32_t)g.nrows * (uint32_t)a.cpitch;   // entries per view
+  const uint32_t rowB = (uint32_t)a.cpitch * kEntry;                 // bytes per row
 
-  // Buffer resources based at the slab's first view (keeps offsets < 4 GiB).
-  const char* fb = static_cast<const char*>(a.filt) + (size_t)vr.x * vstride * kElem;
+  // Buffer resource based at the slab's first view (keeps offsets < 4 GiB;
+  // the planner checks that a slab's views fit).
+  const char* fb = static_cast<const char*>(a.filt) + (size_t)vr.x * vstride * kEntry;
   const uint32_t fbytes =
-      (uint32_t)min((size_t)(vr.y - vr.x) * vstride * kElem, (size_t)0xFFFFFFF0u);
+      (uint32_t)min((size_t)(vr.y - vr.x) * vstride * kEntry, (size_t)0xFFFFFFF0u);
   const __amdgpu_buffer_rsrc_t rf =
       __builtin_amdgcn_make_buffer_rsrc((void*)fb, 0, fbytes, kRsrcWord3);
-  constexpr bool kScaled = PREC != FCT_PRECISION_FP32;
-  const void* sb = kScaled ? (const void*)(a.row_scale + (size_t)vr.x * g.nrows) : (const void*)fb;
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)sb, 0, kScaled ? (uint32_t)((vr.y - vr.x) * g.nrows * 4) : 0u, kRsrcWord3);
 
   float acc[TZ];
   int cnt[TZ];
@@ -240,48 +322,34 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
         const int ilo = max(max(0, (int)floorf(fa) - 1), MODE == kBpMatched ? 0 : wr.x - s0);
         const int ihi = min(min(nz - 1, (int)ceilf(fbb) + 1),
                             MODE == kBpMatched ? nz - 1 : wr.y - s0);
-        if (ilo > ihi) continue;
+        if (ilo > ihi) {
+          if (can_stop && (up ? ilo > nz - 1 : ihi < 0)) break;
+          continue;
+        }
         ilo_t = min(ilo_t, ilo);
         ihi_t = max(ihi_t, ihi);
         const float row0 = fmaf(kbs - ztv, invLh, g.cr);   // fct_bp_row0
         ChanInterp<PREC> ci;
-        ci.init((uint32_t)(v - vr.x) * vstride * kElem, pf, wp);
-        const uint32_t sbase = (uint32_t)(v - vr.x) * (uint32_t)g.nrows * 4u;
+        ci.init((uint32_t)(v - vr.x) * vstride * kEntry, pf, wp);
+        if (ilo == 0 && ihi == nz - 1) {
+          // Every slice of the slab (the common case): no per-slice branch,
+          // and the taps of all slices are loaded before the first is used.
+          // With nz < TZ the extra slices repeat the last one and are not
+          // stored.
+          BpTap<PREC> tap[TZ];
 #pragma unroll
-        for (int i = 0; i < TZ; ++i) {
-          if (i < ilo || i > ihi) continue;
-          const float qi = fmaf(zs[i], invLh, row0);       // fct_bp_row
-          const float qfl = floorf(qi);
-          const int iq = (int)qfl;
-          bool sel;
-          if (CENTERED)
-            sel = chan_ok && (uint32_t)iq <= (uint32_t)(g.nrows - 2);
-          else
-            sel = chan_ok && !(fabsf(qi - g.cr) > g.hr) && qfl >= 0.0f &&
-                  qfl <= (float)(g.nrows - 2);
-          if (MODE == kBpCount) {
-            cnt[i] += sel ? 1 : 0;
-            continue;
-          }
-          // 24-bit multiply: exact for selected rows (< 2^24); other lanes
-          // read an arbitrary in- or out-of-range offset and get weight 0.
-          const uint32_t off = ci.base + __umul24((uint32_t)iq, rowB);
-          float lo = ci.row(rf, off, 0);
-          float hi = ci.row(rf, off, (int)rowB);
-          if (kScaled) {
-            const auto sc = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(sbase + (uint32_t)iq * 4u),
-                                                                 0, 0);
-            lo *= __uint_as_float(sc[0]);
-            hi *= __uint_as_float(sc[1]);
-          }
-          const float val = fmaf(qi - qfl, hi - lo, lo);
-          if (MODE == kBpMatched) {   // plain sum over the views, weight R / l
-            num[i] = fmaf(sel ? fR : 0.0f, val, num[i]);
-          } else {
-            const float W = sel ? fct_w_of_row(qi, g) : 0.0f;
-            num[i] = fmaf(W, val, num[i]);
-            den[i] += W;
-          }
+          for (int i = 0; i < TZ; ++i)
+            tap[i] = bp_fetch<PREC, MODE, CENTERED>(zs[i], invLh, row0, ci, rf, rowB, chan_ok, g);
+#pragma unroll
+          for (int i = 0; i < TZ; ++i) bp_add<PREC, MODE>(tap[i], ci, fR, g, num[i], den[i], cnt[i]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < TZ; ++i)
+            if (i >= ilo && i <= ihi) {
+              const BpTap<PREC> tp =
+                  bp_fetch<PREC, MODE, CENTERED>(zs[i], invLh, row0, ci, rf, rowB, chan_ok, g);
+              bp_add<PREC, MODE>(tp, ci, fR, g, num[i], den[i], cnt[i]);
+            }
         }
       }
     }
@@ -297,9 +365,12 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   }
   if (!inb) return;
   if (MODE == kBpAccumulate || MODE == kBpMatched) {
+    // FP16 / BF16 projections carry one storage scale per scan.
+    const float out_scale =
+        PREC == FCT_PRECISION_FP32 ? a.scale : a.scale * ld_uniform(a.filt_scale);
 #pragma unroll
     for (int i = 0; i < TZ; ++i)
-      if (i < nz) a.native[((size_t)(s0 + i) * a.ny + iy) * a.nx + ix] = acc[i] * a.scale;
+      if (i < nz) a.native[((size_t)(s0 + i) * a.ny + iy) * a.nx + ix] = acc[i] * out_scale;
   } else if (MODE == kBpCount) {
     unsigned long long tot = 0;
 #pragma unroll

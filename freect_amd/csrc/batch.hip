@@ -77,7 +77,7 @@ int upload_jobs(fct_wfbp_gpu_batch* b) {
     fct_wfbp_gpu_plan* p = b->plans[j];
     jobs[j] = fct::plan_bp_args(p);
     jobs[j].filt = p->d_filtered;
-    jobs[j].row_scale = p->d_row_scale;
+    jobs[j].filt_scale = p->d_scale;
     jobs[j].native = p->d_native;
     start[j] = (int32_t)total;
     total += (long long)jobs[j].tiles_x * jobs[j].tiles_y * jobs[j].nslabs;
@@ -122,7 +122,7 @@ int enqueue_backproject(fct_wfbp_gpu_batch* b) {
 int enqueue_rebin_filter(fct_wfbp_gpu_batch* b) {
   for (size_t j = 0; j < b->plans.size(); ++j) {
     fct_wfbp_gpu_plan* p = b->plans[j];
-    const int st = fct_wfbp_gpu_rebin_filter(p, b->raw[j], p->d_filtered, p->d_row_scale);
+    const int st = fct_wfbp_gpu_rebin_filter(p, b->raw[j], p->d_filtered, p->d_scale);
     if (st != FCT_OK) return st;
   }
   return FCT_OK;
@@ -203,11 +203,11 @@ FCT_API int fct_wfbp_gpu_batch_create(fct_wfbp_gpu_ctx* ctx, int32_t n_jobs,
     fct_wfbp_gpu_plan* p = b->plans[j];
     const size_t nat = (size_t)p->n_native * p->recon.ny * p->recon.nx * sizeof(float);
     hipError_t e = hipMalloc(&p->d_filtered, p->filtered_bytes());
-    if (e == hipSuccess && p->row_scale_elems())
-      e = hipMalloc(reinterpret_cast<void**>(&p->d_row_scale), p->row_scale_elems() * 4);
+    if (e == hipSuccess && p->scale_elems())
+      e = hipMalloc(reinterpret_cast<void**>(&p->d_scale), p->scale_elems() * 4);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->d_native), nat);
     if (e != hipSuccess) return fail(hip_status(ctx, "batch buffers", e));
-    b->device_bytes += (int64_t)(p->filtered_bytes() + p->row_scale_elems() * 4 + nat);
+    b->device_bytes += (int64_t)(p->filtered_bytes() + p->scale_elems() * 4 + nat);
   }
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_jobs), n_jobs * sizeof(fct::BpArgs));
   if (e == hipSuccess)

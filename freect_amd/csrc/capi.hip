@@ -148,7 +148,7 @@ int validate(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* s, const fct_recon_
 void free_plan_buffers(fct_wfbp_gpu_plan* p) {
   void* bufs[] = {p->d_xs,       p->d_ys,        p->d_cs,        p->d_zs,       p->d_zt,
                   p->d_slab_vrange, p->d_rebin,  p->d_Hn,        p->d_tw,       p->d_thk_off,
-                  p->d_thk_idx,  p->d_thk_w,     p->d_filtered,  p->d_row_scale, p->d_native,
+                  p->d_thk_idx,  p->d_thk_w,     p->d_filtered,  p->d_scale, p->d_native,
                   p->d_raw_stage, p->d_vol_stage, p->d_gbar, p->d_thkT_off, p->d_thkT_idx,
                   p->d_thkT_w, p->d_wrange};
   for (void* b : bufs)
@@ -173,7 +173,8 @@ bool slab_view_ranges(const fct_wfbp_gpu_plan* p, const std::vector<double>& zn,
     const double lo = std::floor(va) - 2, hi = std::ceil(vb) + 3;
     vr[sl].x = (int)std::max(0.0, std::min(lo, (double)p->nviews));
     vr[sl].y = (int)std::max(0.0, std::min(hi, (double)p->nviews));
-    if ((size_t)(vr[sl].y - vr[sl].x) * p->nrows * p->cpitch * 4 >= ((size_t)1 << 32))
+    if ((size_t)(vr[sl].y - vr[sl].x) * p->nrows * p->cpitch * p->filtered_entry_bytes() >=
+        ((size_t)1 << 32))
       return false;
   }
   return true;
@@ -380,7 +381,11 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   // Exact view-window slice ranges per (chunk, view): the slices s of the
   // chunk with fct_bp_in_window(zt, zs) true form one interval (the test is
   // monotone in zs); the same FP32 values and test as the kernels' per-sample
-  // form, evaluated once here.
+  // form, evaluated once here.  A window that holds no slice of the chunk is
+  // stored as the empty interval just before (a0, a0 - 1) or just after
+  // (a1, a1 - 1) the chunk, on the side where it lies, so that both ends stay
+  // monotone in the view index (the lean kernel ends its view loops on that).
+  const double slice_dir = p->recon.end_pos >= p->recon.start_pos ? 1.0 : -1.0;
   std::vector<int2> wrange((size_t)p->nchunks * p->nviews);
   for (int c = 0; c < p->nchunks; ++c) {
     const int a0 = c * chunk, a1 = std::min(p->n_native, a0 + chunk);
@@ -392,6 +397,11 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
           lo = std::min(lo, sidx);
           hi = std::max(hi, sidx);
         }
+      if (hi < 0) {
+        const bool before = ((double)ztv - (double)zs[a0]) * slice_dir < 0.0;
+        lo = before ? a0 : a1;
+        hi = lo - 1;
+      }
       wrange[(size_t)c * p->nviews + v] = make_int2(lo, hi);
     }
   }
@@ -413,6 +423,8 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   // real.  1/nfft (cuda_kernels.cuh:56-57) is folded in.
   const std::vector<double> h =
       filter_taps(p->nfft, p->ds, r.filter_kind, r.filter_c, r.filter_a);
+  p->hsum = 0.0;   // bound |y| <= max|x| * sum|h| of the FP16 / BF16 storage scale
+  for (double v : h) p->hsum += std::fabs(v);
   std::vector<std::complex<double>> Hc(p->nfft);
   for (int k = 0; k < p->nfft; ++k) {
     const int lag = k < p->nfft / 2 ? k : k - p->nfft;
@@ -509,7 +521,7 @@ FCT_API int fct_wfbp_gpu_plan_info(const fct_wfbp_gpu_plan* p, fct_plan_info* in
   info->channel_pitch = p->cpitch;
   info->raw_elems = p->raw_elems();
   info->filtered_bytes = (int64_t)p->filtered_bytes();
-  info->row_scale_elems = (int64_t)p->row_scale_elems();
+  info->scale_elems = (int64_t)p->scale_elems();
   info->native_elems = (int64_t)p->n_native * p->recon.ny * p->recon.nx;
   info->volume_elems = (int64_t)p->n_req * p->recon.ny * p->recon.nx;
   info->collimated_slice_width = p->csw;
@@ -531,15 +543,18 @@ FCT_API int fct_wfbp_gpu_set_stream(fct_wfbp_gpu_plan* p, void* stream) {
 }
 
 FCT_API int fct_wfbp_gpu_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, void* filtered,
-                                      float* row_scale) {
+                                      float* scale) {
   if (!p || !raw || !filtered) return FCT_ERR_INVALID_ARGUMENT;
-  if (p->opts.precision != FCT_PRECISION_FP32 && !row_scale)
-    return set_err(p->ctx, "row_scale is required for FP16/BF16"), FCT_ERR_INVALID_ARGUMENT;
+  if (p->opts.precision != FCT_PRECISION_FP32 && !scale)
+    return set_err(p->ctx, "the scale buffer is required for FP16/BF16"),
+           FCT_ERR_INVALID_ARGUMENT;
   HIP_TRY(p->ctx, hipSetDevice(p->device));
   fct::RebinFilterArgs a{};
   a.raw = raw;
   a.out = filtered;
-  a.row_scale = row_scale;
+  a.scale = scale;
+  a.hsum = (float)p->hsum;
+  a.raw_elems = p->raw_elems();
   a.tab = p->d_rebin;
   a.Hn = p->d_Hn;
   a.tw = p->d_tw;
@@ -561,14 +576,15 @@ FCT_API int fct_wfbp_gpu_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, vo
 }
 
 FCT_API int fct_wfbp_gpu_backproject(fct_wfbp_gpu_plan* p, const void* filtered,
-                                     const float* row_scale, float* native) {
+                                     const float* scale, float* native) {
   if (!p || !filtered || !native) return FCT_ERR_INVALID_ARGUMENT;
-  if (p->opts.precision != FCT_PRECISION_FP32 && !row_scale)
-    return set_err(p->ctx, "row_scale is required for FP16/BF16"), FCT_ERR_INVALID_ARGUMENT;
+  if (p->opts.precision != FCT_PRECISION_FP32 && !scale)
+    return set_err(p->ctx, "the scale buffer is required for FP16/BF16"),
+           FCT_ERR_INVALID_ARGUMENT;
   HIP_TRY(p->ctx, hipSetDevice(p->device));
   fct::BpArgs a = bp_args(p);
   a.filt = filtered;
-  a.row_scale = row_scale;
+  a.filt_scale = scale;
   a.native = native;
   HIP_TRY(p->ctx, fct::launch_backproject(a, p->tile_z, p->variant, p->opts.precision,
                                           fct::kBpAccumulate, p->stream));
@@ -740,9 +756,9 @@ FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* p, const float* raw, flo
   const size_t nat_bytes = (size_t)p->n_native * p->recon.ny * p->recon.nx * sizeof(float);
   const size_t vol_bytes = (size_t)p->n_req * p->recon.ny * p->recon.nx * sizeof(float);
   if (!p->d_filtered) HIP_TRY(p->ctx, hipMalloc(&p->d_filtered, p->filtered_bytes()));
-  if (p->row_scale_elems() && !p->d_row_scale)
-    HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_row_scale),
-                              p->row_scale_elems() * sizeof(float)));
+  if (p->scale_elems() && !p->d_scale)
+    HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_scale),
+                              p->scale_elems() * sizeof(float)));
   if (!p->d_native) HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_native), nat_bytes));
   const float* d_raw = raw;
   float* d_vol = volume;
@@ -753,9 +769,9 @@ FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* p, const float* raw, flo
     d_raw = p->d_raw_stage;
     d_vol = p->d_vol_stage;
   }
-  int st = fct_wfbp_gpu_rebin_filter(p, d_raw, p->d_filtered, p->d_row_scale);
+  int st = fct_wfbp_gpu_rebin_filter(p, d_raw, p->d_filtered, p->d_scale);
   if (st) return st;
-  st = fct_wfbp_gpu_backproject(p, p->d_filtered, p->d_row_scale, p->d_native);
+  st = fct_wfbp_gpu_backproject(p, p->d_filtered, p->d_scale, p->d_native);
   if (st) return st;
   st = fct_wfbp_gpu_thicken(p, p->d_native, d_vol);
   if (st) return st;

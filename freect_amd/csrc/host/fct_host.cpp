@@ -455,10 +455,15 @@ int SyntheticDataSet::initialize() {
   }
   m_workload = rest;
   if (!fct_synthetic_workload(m_workload, seed, &m_geom, &m_recon, &m_i0, &m_ellipsoids, &m_seed)) {
-    m_last_error = "unknown synthetic workload '" + m_workload + "' (small, full, adjoint)";
+    m_last_error = "unknown synthetic workload '" + m_workload + "' (small, full, adjoint, ffs)";
     return FCT_ERR_INVALID_ARGUMENT;
   }
   m_manufacturer = "SYNTHETIC";
+  // The DICOM-CT-PD names the reader maps back (DicomDataSet.cpp:193, 285).
+  m_flying_focal_spot_mode = m_geom.ffs_mode == FCT_FFS_PHI_Z ? "FFSXYZ"
+                             : m_geom.ffs_mode == FCT_FFS_PHI ? "FFSXY"
+                             : m_geom.ffs_mode == FCT_FFS_Z   ? "FFSZ"
+                                                              : "FFSNONE";
   m_detector_rows = (uint16_t)m_geom.n_rows;
   m_detector_channels = (uint16_t)m_geom.n_channels;
   m_detector_transverse_spacing = (float)m_geom.channel_spacing;

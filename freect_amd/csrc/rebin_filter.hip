@@ -14,8 +14,11 @@
 // ncols <= N/2 samples, so the circular product equals the linear
 // convolution y[i] = sum_j h[i-j] x[j], exactly what the reference computes.
 //
-// Storage: FP32, or FP16/BF16 with a power-of-two scale per line chosen so
-// that max|y|/scale <= 2^15.
+// Storage: FP32 rows, or FP16/BF16 quad entries (wfbp_internal.h) divided by
+// one power-of-two scale per scan.  The scale comes from a bound known before
+// any line is filtered: |y| <= max|raw| * sum|h_k| (the rebinning is a convex
+// combination of raw samples), with max|raw| from an abs-max pass over the
+// raw scan.
 #include <hip/hip_bf16.h>
 #include <algorithm>
 #include <hip/hip_fp16.h>
@@ -74,14 +77,90 @@ __device__ FctCplx* fft_forward(FctCplx* a, FctCplx* b, const FctCplx* __restric
   return a;
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
+// Scale of FP16 / BF16 storage from the abs-max pass: the power of two s with
+// bound / s < 2^15, bound = max|raw| * sum|h| * (1 + 2^-7) (the margin covers
+// the FP32 transform's round-off).  1 for an all-zero (or non-finite) scan.
+__device__ __forceinline__ float storage_scale(const RebinFilterArgs& a) {
+  const float m = __uint_as_float(reinterpret_cast<const uint32_t*>(a.scale)[1]);
+  const float bound = m * a.hsum * 1.0078125f;
+  if (!(bound > 0.0f) || !(bound <= 3.0e38f)) return 1.0f;
+  int e = 0;
+  frexpf(bound, &e);   // bound < 2^e
+  return ldexpf(1.0f, e - 15);
+}
+
+template <class T>
+__device__ __forceinline__ T to_store(float v);
+template <>
+__device__ __forceinline__ __half to_store<__half>(float v) { return __float2half_rn(v); }
+template <>
+__device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(float v) {
+  return __float2bfloat16(v);
+}
+
+// Value h of line (view, row r) at channel c into its four quad slots:
+// (r, c).0, (r, c-1).1, (r-1, c).2 and (r-1, c-1).3 of the same view; the
+// slots that would hold channel cpitch or row nrows are zeroed.  e is the
+// entry index of (view, r, c).  Every slot of the buffer is written by exactly
+// one line, so concurrent lines never race.
+template <class T>
+__device__ __forceinline__ void quad_store(T* __restrict__ q, size_t e, int r, int c, int nrows,
+                                           int cpitch, T h) {
+  const T z = to_store<T>(0.0f);
+  const bool last_c = c == cpitch - 1;
+  q[e * 4 + 0] = h;
+  if (c > 0) q[(e - 1) * 4 + 1] = h;
+  if (last_c) q[e * 4 + 1] = z;
+  if (r > 0) {
+    const size_t u = e - (size_t)cpitch;
+    q[u * 4 + 2] = h;
+    if (c > 0) q[(u - 1) * 4 + 3] = h;
+    if (last_c) q[u * 4 + 3] = z;
+  }
+  if (r == nrows - 1) {
+    q[e * 4 + 2] = z;
+    q[e * 4 + 3] = z;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void quad_store_lines(const RebinFilterArgs& a, size_t o0, size_t o1,
+                                                 int r0, int r1, bool has1, int c, float y0,
+                                                 float y1) {
+  T* q = static_cast<T*>(a.out);
+  quad_store<T>(q, o0 + c, r0, c, a.nrows, a.cpitch, to_store<T>(y0));
+  if (has1) quad_store<T>(q, o1 + c, r1, c, a.nrows, a.cpitch, to_store<T>(y1));
+}
+
+// max|x| over the raw scan as FP32 bits in scale[1] (non-negative floats
+// order like their bit patterns).
+__global__ void scale_reset_kernel(float* scale) {
+  if (threadIdx.x == 0) reinterpret_cast<uint32_t*>(scale)[1] = 0u;
+}
+
+template <bool kVec>
+__global__ __launch_bounds__(256) void absmax_kernel(const float* __restrict__ x, int64_t n,
+                                                     float* scale) {
+  uint32_t m = 0;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (kVec) {
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const int64_t n4 = n >> 2;
+    for (; i < n4; i += stride) {
+      const float4 v = x4[i];
+      m = max(m, max(max(__float_as_uint(fabsf(v.x)), __float_as_uint(fabsf(v.y))),
+                     max(__float_as_uint(fabsf(v.z)), __float_as_uint(fabsf(v.w)))));
+    }
+    i = 4 * n4 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  }
+  for (; i < n; i += stride) m = max(m, __float_as_uint(fabsf(x[i])));
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(reinterpret_cast<uint32_t*>(scale) + 1, m);
 }
 
 __global__ __launch_bounds__(kRfThreads) void rebin_filter_kernel(RebinFilterArgs a) {
   extern __shared__ FctCplx lds[];
-  __shared__ float red[2][kRfThreads / 64];
   const int N = a.nfft;
   FctCplx* buf0 = lds;
   FctCplx* buf1 = lds + N;
@@ -137,50 +216,16 @@ __global__ __launch_bounds__(kRfThreads) void rebin_filter_kernel(RebinFilterArg
     }
     return;
   }
-  // Per-line power-of-two scale: max|y| / scale <= 2^15.
-  float m0 = 0.0f, m1 = 0.0f;
-  for (int c = threadIdx.x; c < a.ncols; c += kRfThreads) {
-    m0 = fmaxf(m0, fabsf(Y[c].x));
-    m1 = fmaxf(m1, fabsf(Y[c].y));
-  }
-  m0 = wave_max(m0);
-  m1 = wave_max(m1);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[0][w] = m0;
-    red[1][w] = m1;
-  }
-  __syncthreads();
-  m0 = red[0][0];
-  m1 = red[1][0];
-  for (int i = 1; i < kRfThreads / 64; ++i) {
-    m0 = fmaxf(m0, red[0][i]);
-    m1 = fmaxf(m1, red[1][i]);
-  }
-  int e0 = 0, e1 = 0;
-  frexpf(m0, &e0);
-  frexpf(m1, &e1);
-  const float s0 = m0 > 0.0f ? ldexpf(1.0f, e0 - 15) : 1.0f;
-  const float s1 = m1 > 0.0f ? ldexpf(1.0f, e1 - 15) : 1.0f;
-  const float i0 = 1.0f / s0, i1 = 1.0f / s1;   // exact: powers of two
-  if (threadIdx.x == 0) {
-    a.row_scale[l0] = s0;
-    if (has1) a.row_scale[l1] = s1;
-  }
-  if (a.precision == FCT_PRECISION_FP16) {
-    __half* out = static_cast<__half*>(a.out);
-    for (int c = threadIdx.x; c < a.cpitch; c += kRfThreads) {
-      const bool in = c < a.ncols;
-      out[o0 + c] = __float2half_rn(in ? Y[c].x * i0 : 0.0f);
-      if (has1) out[o1 + c] = __float2half_rn(in ? -Y[c].y * i1 : 0.0f);
-    }
-  } else {
-    __hip_bfloat16* out = static_cast<__hip_bfloat16*>(a.out);
-    for (int c = threadIdx.x; c < a.cpitch; c += kRfThreads) {
-      const bool in = c < a.ncols;
-      out[o0 + c] = __float2bfloat16(in ? Y[c].x * i0 : 0.0f);
-      if (has1) out[o1 + c] = __float2bfloat16(in ? -Y[c].y * i1 : 0.0f);
-    }
+  const float s = storage_scale(a);
+  const float is = 1.0f / s;   // exact: a power of two
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.scale[0] = s;
+  for (int c = threadIdx.x; c < a.cpitch; c += kRfThreads) {
+    const bool in = c < a.ncols;
+    const float y0 = in ? Y[c].x * is : 0.0f, y1 = in ? -Y[c].y * is : 0.0f;
+    if (a.precision == FCT_PRECISION_FP16)
+      quad_store_lines<__half>(a, o0, o1, r0, r1, has1, c, y0, y1);
+    else
+      quad_store_lines<__hip_bfloat16>(a, o0, o1, r0, r1, has1, c, y0, y1);
   }
 }
 
@@ -245,7 +290,6 @@ __global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs
   constexpr bool kPingPong = N <= 2048;
   constexpr int kSlots = N + N / 8;
   __shared__ FctCplx lds[kPingPong ? 2 * kSlots : kSlots];
-  __shared__ float red[2][T / 64];
   const int t = threadIdx.x;
   const FctCplx* tw = reinterpret_cast<const FctCplx*>(a.tw);
 
@@ -256,6 +300,12 @@ __global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs
   float hn[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) hn[m] = a.Hn[t + m * T];
+  float is = 1.0f;   // 1 / storage scale (FP16 / BF16)
+  if (a.precision != FCT_PRECISION_FP32 && !a.rebin_only) {
+    const float s = storage_scale(a);
+    is = 1.0f / s;   // exact: a power of two
+    if (blockIdx.x == 0 && t == 0) a.scale[0] = s;
+  }
 
   const int npairs = (a.nlines + 1) >> 1;
   for (int pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
@@ -322,56 +372,17 @@ __global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs
       }
       continue;
     }
-    // Per-line power-of-two scale: max|y| / scale <= 2^15.
-    float m0 = 0.0f, m1 = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      if (t + m * T < a.ncols) {
-        const FctCplx y = w[fct_fft_final_reg<N>(m)];
-        m0 = fmaxf(m0, fabsf(y.x));
-        m1 = fmaxf(m1, fabsf(y.y));
-      }
-    }
-    m0 = wave_max(m0);
-    m1 = wave_max(m1);
-    if ((t & 63) == 0) {
-      red[0][t >> 6] = m0;
-      red[1][t >> 6] = m1;
-    }
-    __syncthreads();
-    m0 = red[0][0];
-    m1 = red[1][0];
-#pragma unroll
-    for (int i = 1; i < T / 64; ++i) {
-      m0 = fmaxf(m0, red[0][i]);
-      m1 = fmaxf(m1, red[1][i]);
-    }
-    int e0 = 0, e1 = 0;
-    frexpf(m0, &e0);
-    frexpf(m1, &e1);
-    const float s0 = m0 > 0.0f ? ldexpf(1.0f, e0 - 15) : 1.0f;
-    const float s1 = m1 > 0.0f ? ldexpf(1.0f, e1 - 15) : 1.0f;
-    const float i0 = 1.0f / s0, i1 = 1.0f / s1;   // exact: powers of two
-    if (t == 0) {
-      a.row_scale[l0] = s0;
-      if (has1) a.row_scale[l1] = s1;
-    }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const int c = t + m * T;
       if (c >= a.cpitch) continue;
       const FctCplx y = w[fct_fft_final_reg<N>(m)];
       const bool in = c < a.ncols;
-      const float y0 = in ? y.x * i0 : 0.0f, y1 = in ? -y.y * i1 : 0.0f;
-      if (a.precision == FCT_PRECISION_FP16) {
-        __half* out = static_cast<__half*>(a.out);
-        out[o0 + c] = __float2half_rn(y0);
-        if (has1) out[o1 + c] = __float2half_rn(y1);
-      } else {
-        __hip_bfloat16* out = static_cast<__hip_bfloat16*>(a.out);
-        out[o0 + c] = __float2bfloat16(y0);
-        if (has1) out[o1 + c] = __float2bfloat16(y1);
-      }
+      const float y0 = in ? y.x * is : 0.0f, y1 = in ? -y.y * is : 0.0f;
+      if (a.precision == FCT_PRECISION_FP16)
+        quad_store_lines<__half>(a, o0, o1, r0, r1, has1, c, y0, y1);
+      else
+        quad_store_lines<__hip_bfloat16>(a, o0, o1, r0, r1, has1, c, y0, y1);
     }
   }
 }
@@ -397,6 +408,19 @@ hipError_t launch_reg(const RebinFilterArgs& a, hipStream_t s) {
 
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s) {
   if (a.nlines <= 0) return hipSuccess;
+  if (a.precision != FCT_PRECISION_FP32 && !a.rebin_only) {
+    if (!a.scale || a.lines_in) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(scale_reset_kernel, dim3(1), dim3(64), 0, s, a.scale);
+    const int64_t n = a.raw_elems;
+    const int64_t want = (n / 4 + 255) / 256;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 8LL * std::max(a.ncu, 1)));
+    if ((reinterpret_cast<uintptr_t>(a.raw) & 15) == 0)
+      hipLaunchKernelGGL(absmax_kernel<true>, dim3(grid), dim3(256), 0, s, a.raw, n, a.scale);
+    else
+      hipLaunchKernelGGL(absmax_kernel<false>, dim3(grid), dim3(256), 0, s, a.raw, n, a.scale);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   switch (a.nfft) {
     case 512: return launch_reg<512>(a, s);
     case 1024: return launch_reg<1024>(a, s);

@@ -23,11 +23,23 @@ struct RebinDev {
   int32_t mmax, pad;
 };
 
+// Filtered projections in HBM.
+//  * FP32: plain rows, [view][row][cpitch] floats.
+//  * FP16 / BF16: bilinear "quad" entries, [view][row][cpitch][4] 16-bit
+//    values, entry (r, c) = (y[r][c], y[r][c+1], y[r+1][c], y[r+1][c+1]) / s,
+//    so that one 8-byte load fetches the 2 x 2 footprint of a sample.  Slots
+//    beyond the detector (c + 1 = cpitch, r + 1 = nrows) hold 0.  s is one
+//    power of two per scan, chosen from a bound on |y| so that |y| / s < 2^15
+//    (no overflow); it is kept in scale[0] of the scan's scale buffer.
+constexpr int kScaleElems = 2;   // [0] s, [1] max |raw| bits (rebin+filter scratch)
+
 // Everything the rebin + filter kernel reads.
 struct RebinFilterArgs {
   const float* raw;          // [view][row][ncols]
-  void* out;                 // [view][row][cpitch]
-  float* row_scale;          // [view][row] or null
+  void* out;                 // filtered projections (layout above)
+  float* scale;              // [kScaleElems] (FP16 / BF16) or null
+  float hsum;                // sum |h_k| of the filter taps (bound on |y| / max|x|)
+  int64_t raw_elems;         // raw floats (the abs-max pass)
   const RebinDev* tab;       // [ncols]
   const float* Hn;           // [nfft] real filter spectrum / nfft
   const float2* tw;          // [nfft] exp(-2 pi i k / nfft)
@@ -44,8 +56,8 @@ struct RebinFilterArgs {
 
 // Everything the backprojection kernel reads.
 struct BpArgs {
-  const void* filt;          // [view][row][cpitch] FP32 / FP16 / BF16
-  const float* row_scale;    // [view][row] (FP16/BF16 only)
+  const void* filt;          // filtered projections (layout above)
+  const float* filt_scale;   // scale buffer of FP16 / BF16 projections
   float* native;             // [slice][y][x]
   int32_t* counts;           // count mode: [slice][y][x]
   unsigned long long* total; // count mode
@@ -178,7 +190,7 @@ struct fct_wfbp_gpu_plan {
 
   // Intermediate buffers for fct_wfbp_gpu_reconstruct (allocated on first use).
   void* d_filtered = nullptr;
-  float* d_row_scale = nullptr;
+  float* d_scale = nullptr;
   float* d_native = nullptr;
   float* d_raw_stage = nullptr;
   float* d_vol_stage = nullptr;
@@ -190,11 +202,14 @@ struct fct_wfbp_gpu_plan {
   int64_t raw_elems() const {
     return (int64_t)raw_scan.n_views * raw_scan.n_rows * raw_scan.n_channels;
   }
+  size_t filtered_entry_bytes() const {   // FP32 value, or 4 x 16-bit quad
+    return opts.precision == FCT_PRECISION_FP32 ? 4 : 8;
+  }
   size_t filtered_bytes() const {
-    const size_t el = opts.precision == FCT_PRECISION_FP32 ? 4 : 2;
-    return (size_t)nviews * nrows * cpitch * el;
+    return (size_t)nviews * nrows * cpitch * filtered_entry_bytes();
   }
-  size_t row_scale_elems() const {
-    return opts.precision == FCT_PRECISION_FP32 ? 0 : (size_t)nviews * nrows;
+  size_t scale_elems() const {
+    return opts.precision == FCT_PRECISION_FP32 ? 0 : (size_t)fct::kScaleElems;
   }
+  double hsum = 0;                  // sum |h_k| of the filter taps
 };

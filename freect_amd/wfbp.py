@@ -54,7 +54,7 @@ class ExecOptionsC(C.Structure):
 class PlanInfoC(C.Structure):
     _fields_ = [("n_slices_native", C.c_int32), ("n_slices_requested", C.c_int32),
                 ("nfft", C.c_int32), ("channel_pitch", C.c_int32), ("raw_elems", C.c_int64),
-                ("filtered_bytes", C.c_int64), ("row_scale_elems", C.c_int64),
+                ("filtered_bytes", C.c_int64), ("scale_elems", C.c_int64),
                 ("native_elems", C.c_int64), ("volume_elems", C.c_int64),
                 ("collimated_slice_width", C.c_double), ("acquisition_fov", C.c_double),
                 ("z_rot", C.c_double), ("tile_z", C.c_int32), ("bp_variant", C.c_int32),
@@ -262,8 +262,8 @@ class Plan:
         import torch
         i = self.info
         filt = torch.empty(i.filtered_bytes, dtype=torch.uint8, device=self._dev())
-        scale = (torch.empty(i.row_scale_elems, dtype=torch.float32, device=self._dev())
-                 if i.row_scale_elems else None)
+        scale = (torch.empty(i.scale_elems, dtype=torch.float32, device=self._dev())
+                 if i.scale_elems else None)
         return filt, scale
 
     def alloc_native(self):
@@ -278,12 +278,30 @@ class Plan:
         return torch.empty((i.n_slices_requested, self.recon.ny, self.recon.nx),
                            dtype=torch.float32, device=self._dev())
 
-    def filtered_view(self, filt):
-        """The filtered buffer as a [view][row][channel_pitch] tensor."""
+    def filtered_view(self, filt, scale=None):
+        """The filtered projections y as a [view][row][channel_pitch] tensor.
+
+        FP32: a view of the buffer.  FP16 / BF16: the (r, c) slot of every
+        quad entry, in the storage type, multiplied by the scan's storage
+        scale (scale[0]) when the scale buffer is given."""
         import torch
         dt = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}[self.precision]
         i = self.info
-        return filt.view(dt).view(i.n_parallel_views, i.n_filtered_rows, i.channel_pitch)
+        shape = (i.n_parallel_views, i.n_filtered_rows, i.channel_pitch)
+        if self.precision == "fp32":
+            return filt.view(dt).view(shape)
+        y = filt.view(dt).view(shape + (4,))[..., 0]
+        return y if scale is None else y.float() * scale[0]
+
+    def filtered_quads(self, filt):
+        """FP16 / BF16 only: the quad entries as [view][row][channel_pitch][4]
+        (y[r][c], y[r][c+1], y[r+1][c], y[r+1][c+1]) in units of the scale."""
+        import torch
+        if self.precision == "fp32":
+            raise FctError("FP32 projections are stored as plain rows")
+        dt = {"fp16": torch.float16, "bf16": torch.bfloat16}[self.precision]
+        i = self.info
+        return filt.view(dt).view(i.n_parallel_views, i.n_filtered_rows, i.channel_pitch, 4)
 
     # -- stages -------------------------------------------------------------
     def set_stream(self, stream=None):

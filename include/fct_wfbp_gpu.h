@@ -113,8 +113,9 @@ typedef struct {
   int32_t nfft;                /* 2^ceil(log2(2*ncols)) (ReconstructionMachine.cu:54) */
   int32_t channel_pitch;       /* padded channels per filtered row */
   int64_t raw_elems;           /* n_views*n_rows*n_channels floats, [view][row][ch] */
-  int64_t filtered_bytes;      /* bytes of the filtered buffer, [view][row][channel_pitch] */
-  int64_t row_scale_elems;     /* floats of the per-(view,row) scale buffer (0 for FP32) */
+  int64_t filtered_bytes;      /* bytes of the filtered buffer: FP32 [view][row][channel_pitch];
+                                  FP16/BF16 [view][row][channel_pitch][4] bilinear quads */
+  int64_t scale_elems;         /* floats of the FP16/BF16 scale buffer (0 for FP32) */
   int64_t native_elems;        /* n_slices_native*ny*nx floats, [slice][y][x] */
   int64_t volume_elems;        /* n_slices_requested*ny*nx floats, [slice][y][x] */
   double collimated_slice_width;
@@ -163,15 +164,19 @@ FCT_API int fct_wfbp_gpu_set_stream(fct_wfbp_gpu_plan* plan, void* stream);
 
 /* Rebin + ramp filter (ReconstructionMachine.cu:141-306, cuda_kernels.cuh:11-77).
  * raw: device [view][row][ch] float32 fan data.  filtered: device buffer of
- * filtered_bytes; row_scale: device buffer of row_scale_elems floats (NULL for
- * FP32).  Asynchronous on the plan's stream. */
+ * filtered_bytes.  FP32 stores the filtered rows y[view][row][ch]; FP16/BF16
+ * store, per (view, row r, channel c), the quad (y[r][c], y[r][c+1],
+ * y[r+1][c], y[r+1][c+1]) / s (0 beyond the detector) with one power-of-two s
+ * per scan that keeps |y| / s < 2^15, and write s to scale[0].  scale: device
+ * buffer of scale_elems floats (NULL for FP32).  Asynchronous on the plan's
+ * stream. */
 FCT_API int fct_wfbp_gpu_rebin_filter(fct_wfbp_gpu_plan* plan, const float* raw,
-                                      void* filtered, float* row_scale);
+                                      void* filtered, float* scale);
 
 /* Helical-weighted backprojection into native slices
  * (ReconstructionMachine.cu:308-385, cuda_kernels.cuh:79-93, 198-287). */
 FCT_API int fct_wfbp_gpu_backproject(fct_wfbp_gpu_plan* plan, const void* filtered,
-                                     const float* row_scale, float* native);
+                                     const float* scale, float* native);
 
 /* Triangular slice thickening (ReconstructionMachine.cu:387-434,
  * cuda_kernels.cuh:289-320). */

@@ -69,6 +69,7 @@ def lib():
         sig = {
             "or_geometry": (C.c_int, [P(OrScan), P(OrRecon), P(OrDerived)]),
             "or_effective_scan": (None, [P(OrScan), P(OrScan)]),
+            "or_weights": (None, [fp, C.c_int, fp, dp]),
             "or_slice_grid": (C.c_int, [C.c_double, C.c_double, C.c_double, dp, C.c_int]),
             "or_filter_kernel": (None, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, dp]),
             "or_rebin": (None, [P(OrScan), fp, dp, C.c_int, C.c_int]),
@@ -116,6 +117,18 @@ def _eff(scan):
     """Parallel-domain functions work on the interleaved detector of an FFS
     scan (ScanGeometry.effective(); checked against or_effective_scan)."""
     return scan.effective() if getattr(scan, "ffs_mode", 0) else scan
+
+
+def weights(q):
+    """The shared FP32 helical weight W(q) (wfbp_math.h) and its FP64
+    definition (cuda_kernels.cuh:79-93) at the FP32 points q."""
+    q = np.ascontiguousarray(q, dtype=np.float32)
+    w32 = np.empty(q.shape, np.float32)
+    w64 = np.empty(q.shape, np.float64)
+    lib().or_weights(q.ctypes.data_as(C.POINTER(C.c_float)), int(q.size),
+                     w32.ctypes.data_as(C.POINTER(C.c_float)),
+                     w64.ctypes.data_as(C.POINTER(C.c_double)))
+    return w32, w64
 
 
 def effective_scan(scan) -> dict:

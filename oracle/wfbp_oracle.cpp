@@ -260,6 +260,14 @@ extern "C" {
 
 void or_effective_scan(const or_scan* s, or_scan* out) { *out = effective(s); }
 
+// The shared FP32 helical weight (wfbp_math.h) and the FP64 definition at q.
+void or_weights(const float* q, int n, float* w32, double* w64) {
+  for (int i = 0; i < n; ++i) {
+    w32[i] = fct_w_of_q(q[i]);
+    w64[i] = w_of_q((double)q[i]);
+  }
+}
+
 int or_geometry(const or_scan* s, const or_recon* r, or_derived* d) {
   if (!s || !d) return -1;
   const Geo G = make_geo(s);

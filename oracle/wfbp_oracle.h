@@ -72,6 +72,8 @@ typedef struct {
 // or_rebin_adjoint, or_project_phantom, or_reconstruct_native and
 // or_adjoint_native takes the interleaved (parallel-domain) geometry.
 void or_effective_scan(const or_scan* s, or_scan* out);
+// Shared FP32 helical weight fct_w_of_q and the FP64 definition at q[0..n).
+void or_weights(const float* q, int n, float* w32, double* w64);
 int or_geometry(const or_scan* s, const or_recon* r, or_derived* d);
 int or_slice_grid(double start, double end, double step, double* z, int cap);
 void or_filter_kernel(int n, double ds, int kind, double c, double a, double* h);

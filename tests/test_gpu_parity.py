@@ -88,10 +88,13 @@ def test_rebin_filter_matches_oracle(ctx, oracle, filt):
 def test_rebin_filter_fft_sizes(ctx, oracle, n_channels):
     # nfft 256 (generic LDS kernel) and 512 / 1024 / 2048 / 4096 (the
     # register-resident kernel, one instantiation each), FP32 against the
-    # FP64 oracle, then FP16 / BF16 storage against the FP32 output: per line
-    # the stored value times its power-of-two scale is within half an ulp of
-    # the format at the line maximum (2^-11 / 2^-8 relative) plus the FFT
-    # round-off.
+    # FP64 oracle, then FP16 / BF16 storage against the FP32 output: the
+    # stored value times the scan's power-of-two scale s is within half an ulp
+    # of the format (2^-11 / 2^-8 relative, FP16's subnormal spacing 2^-24 s
+    # below 2^-14 s) plus the FFT round-off; s is the documented one
+    # (max|y| < 2^15 s, 2^14 s <= max|raw| sum|h| (1 + 2^-7)); and every slot
+    # of the 2 x 2 quads holds the stored (r, c) value of its neighbour, 0
+    # past the last channel of the pitch and past the last row.
     w = S.tiny(n_channels=n_channels, n_rows=4, ppr=96, rotations=2.0)
     sc, rc = w.scan, w.recon
     raw_t = _raw_gpu(ctx, w, 0.0)
@@ -118,13 +121,20 @@ def test_rebin_filter_fft_sizes(ctx, oracle, n_channels):
             assert err < tol
             f32 = got
             continue
-        s = to_np(scale_t).reshape(sc.n_views, sc.n_rows, 1)
-        deq = got * s
-        line_max = np.abs(f32).max(axis=-1, keepdims=True)
-        assert np.all(line_max <= s * 2.0 ** 15 + 1e-30)
-        err = np.abs(deq - f32) - (rel * line_max + tol)
-        say("  %s worst margin %.3g" % (prec, err.max()))
+        s = float(to_np(scale_t)[0])
+        bound = np.abs(raw).max() * np.abs(h).sum() * (1 + 2.0 ** -7)
+        assert s == 2.0 ** round(math.log2(s))
+        assert np.abs(f32).max() < s * 2.0 ** 15 and s * 2.0 ** 14 <= bound * (1 + 1e-6)
+        deq = got.astype(np.float64) * s
+        err = np.abs(deq - f32) - (rel * np.abs(f32) + s * 2.0 ** -24 + tol)
+        say("  %s s %g worst margin %.3g" % (prec, s, err.max()))
         assert np.all(err <= 0.0)
+        q = to_np(p.filtered_quads(filt_t).float())
+        y = q[..., 0]
+        assert np.array_equal(q[:, :, :-1, 1], y[:, :, 1:]) and np.all(q[:, :, -1, 1] == 0)
+        assert np.array_equal(q[:, :-1, :, 2], y[:, 1:, :]) and np.all(q[:, -1, :, 2] == 0)
+        assert np.array_equal(q[:, :-1, :-1, 3], y[:, 1:, 1:])
+        assert np.all(q[:, -1, :, 3] == 0) and np.all(q[:, :, -1, 3] == 0)
 
 
 @pytest.mark.parametrize("cr_shift", [0.0, 0.37])
@@ -270,5 +280,5 @@ def test_error_paths(ctx):
     with pytest.raises(wfbp.FctError, match="bp_variant"):
         wfbp.Plan(ctx, w.scan, w.recon, bp_variant=9)
     p = wfbp.Plan(ctx, w.scan, w.recon, precision="fp16")
-    with pytest.raises(wfbp.FctError, match="row_scale"):
+    with pytest.raises(wfbp.FctError, match="scale buffer"):
         p.backproject(1, None, 1)
