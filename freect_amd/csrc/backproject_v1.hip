@@ -20,8 +20,8 @@
 //     test, one unsigned compare;
 //   * taps: FP32 rows are two 8-byte buffer loads (the second row through the
 //     scalar offset); FP16 / BF16 projections are stored as 2 x 2 quads, one
-//     8-byte load per sample, interpolated across channels with
-//     v_dot2_f32_f16 (FP16), and scaled once per scan at the end; lanes that
+//     8-byte load per sample, interpolated across channels straight from the
+//     FP16 halves (v_fma_mix_f32), and scaled once per scan at the end; lanes that
 //     select nothing read whatever the range-checked buffer returns and get
 //     weight 0 instead of a branch;
 //   * the per-theta~ mean uses v_rcp.
@@ -113,10 +113,11 @@ template <>
 struct ChanInterp<FCT_PRECISION_FP16> {   // quad entries: one 8-byte load
   static constexpr uint32_t kEntry = 8;
   typedef u32x2 Raw;
-  half2_t w2;      // (1 - wp, wp)
+  float wp, omw;   // channel weights wp, 1 - wp
   uint32_t base;
   __device__ void init(uint32_t view_bytes, uint32_t pf, float w) {
-    w2 = half2_t{(_Float16)(1.0f - w), (_Float16)w};
+    wp = w;
+    omw = 1.0f - w;
     base = view_bytes + pf * kEntry;
   }
   __device__ static Raw fetch(__amdgpu_buffer_rsrc_t r, uint32_t off, int) {
@@ -124,10 +125,12 @@ struct ChanInterp<FCT_PRECISION_FP16> {   // quad entries: one 8-byte load
   }
   __device__ float2 rows(const Raw& t) const {
     // Through named scalars: hipcc (ROCm 7.2) folds __builtin_bit_cast of a
-    // vector element such as t[1] to element 0.
+    // vector element such as t[1] to element 0.  The products read the FP16
+    // halves directly (v_fma_mix_f32), FP32 weights and sums.
     const uint32_t r0 = t.x, r1 = t.y;
-    return make_float2(__builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, r0), w2, 0.0f, false),
-                       __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, r1), w2, 0.0f, false));
+    const half2_t h0 = __builtin_bit_cast(half2_t, r0), h1 = __builtin_bit_cast(half2_t, r1);
+    return make_float2(fmaf((float)h0.y, wp, fmaf((float)h0.x, omw, 0.0f)),
+                       fmaf((float)h1.y, wp, fmaf((float)h1.x, omw, 0.0f)));
   }
 };
 
@@ -333,15 +336,20 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
         ci.init((uint32_t)(v - vr.x) * vstride * kEntry, pf, wp);
         if (ilo == 0 && ihi == nz - 1) {
           // Every slice of the slab (the common case): no per-slice branch,
-          // and the taps of all slices are loaded before the first is used.
-          // With nz < TZ the extra slices repeat the last one and are not
-          // stored.
-          BpTap<PREC> tap[TZ];
+          // and the taps of four slices are in flight before the first is
+          // used (four, not TZ: more would cost waves per SIMD).  With
+          // nz < TZ the extra slices repeat the last one and are not stored.
 #pragma unroll
-          for (int i = 0; i < TZ; ++i)
-            tap[i] = bp_fetch<PREC, MODE, CENTERED>(zs[i], invLh, row0, ci, rf, rowB, chan_ok, g);
+          for (int h = 0; h < TZ; h += 4) {
+            BpTap<PREC> tap[4];
 #pragma unroll
-          for (int i = 0; i < TZ; ++i) bp_add<PREC, MODE>(tap[i], ci, fR, g, num[i], den[i], cnt[i]);
+            for (int i = 0; i < 4; ++i)
+              tap[i] = bp_fetch<PREC, MODE, CENTERED>(zs[h + i], invLh, row0, ci, rf, rowB,
+                                                      chan_ok, g);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              bp_add<PREC, MODE>(tap[i], ci, fR, g, num[h + i], den[h + i], cnt[h + i]);
+          }
         } else {
 #pragma unroll
           for (int i = 0; i < TZ; ++i)
@@ -397,8 +405,13 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
 // eight counters have passed their range ends.  A tile is computed by the
 // same code as in bp_v1_kernel, so each job's volume is bitwise identical to
 // its own single-job launch with the same tile_z.
+// (Six waves per SIMD for the 16-bit kernels at TZ <= 8: the persistent loop
+// would otherwise hold the registers of five, and these fit six without
+// spilling.)
 template <int TZ, int PREC, bool CENTERED>
-__global__ __launch_bounds__(64) void bp_batch_kernel(BpBatchArgs b,
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(TZ <= 8 && PREC != FCT_PRECISION_FP32 ? 6 : 1, 8)))
+void bp_batch_kernel(BpBatchArgs b,
                                                       const BpArgs* __restrict__ jobs,
                                                       const int32_t* __restrict__ tile_start) {
   const int xcd = blockIdx.x & 7;
