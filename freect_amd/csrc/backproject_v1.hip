@@ -35,30 +35,44 @@ namespace {
 
 constexpr int kRsrcWord3 = 0x00020000;   // raw buffer, 32-bit data format (gfx9)
 
+// Lane move; rows outside ROW_MASK are left unspecified (the reductions
+// below only read lane 63, whose inputs are always in enabled rows).
 template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp(float v) {
-  return __int_as_float(
-      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+__device__ __forceinline__ int dpp(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, ROW_MASK, 0xF, true);
+}
+
+// Order-preserving map of a float's bits to a signed int (negative floats get
+// their magnitude bits flipped); the map is its own inverse.  Integer min /
+// max fuse with the DPP lane moves, one instruction per step.
+__device__ __forceinline__ int fkey(float x) {
+  const int b = __float_as_int(x);
+  return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
+__device__ __forceinline__ float fkey_inv(int k) {
+  return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF));
 }
 
 // Wave-wide min / max (all 64 lanes must be active), result uniform.
-__device__ __forceinline__ float wave_min(float v) {
-  v = fminf(v, dpp<0xB1, 0xF>(v));    // quad_perm [1,0,3,2]
-  v = fminf(v, dpp<0x4E, 0xF>(v));    // quad_perm [2,3,0,1]
-  v = fminf(v, dpp<0x141, 0xF>(v));   // row_half_mirror
-  v = fminf(v, dpp<0x140, 0xF>(v));   // row_mirror
-  v = fminf(v, dpp<0x142, 0xA>(v));   // row_bcast:15
-  v = fminf(v, dpp<0x143, 0xC>(v));   // row_bcast:31
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+__device__ __forceinline__ float wave_min(float x) {
+  int v = fkey(x);
+  v = min(v, dpp<0xB1, 0xF>(v));    // quad_perm [1,0,3,2]
+  v = min(v, dpp<0x4E, 0xF>(v));    // quad_perm [2,3,0,1]
+  v = min(v, dpp<0x141, 0xF>(v));   // row_half_mirror
+  v = min(v, dpp<0x140, 0xF>(v));   // row_mirror
+  v = min(v, dpp<0x142, 0xA>(v));   // row_bcast:15
+  v = min(v, dpp<0x143, 0xC>(v));   // row_bcast:31
+  return fkey_inv(__builtin_amdgcn_readlane(v, 63));
 }
-__device__ __forceinline__ float wave_max(float v) {
-  v = fmaxf(v, dpp<0xB1, 0xF>(v));
-  v = fmaxf(v, dpp<0x4E, 0xF>(v));
-  v = fmaxf(v, dpp<0x141, 0xF>(v));
-  v = fmaxf(v, dpp<0x140, 0xF>(v));
-  v = fmaxf(v, dpp<0x142, 0xA>(v));
-  v = fmaxf(v, dpp<0x143, 0xC>(v));
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+__device__ __forceinline__ float wave_max(float x) {
+  int v = fkey(x);
+  v = max(v, dpp<0xB1, 0xF>(v));
+  v = max(v, dpp<0x4E, 0xF>(v));
+  v = max(v, dpp<0x141, 0xF>(v));
+  v = max(v, dpp<0x140, 0xF>(v));
+  v = max(v, dpp<0x142, 0xA>(v));
+  v = max(v, dpp<0x143, 0xC>(v));
+  return fkey_inv(__builtin_amdgcn_readlane(v, 63));
 }
 
 __device__ __forceinline__ int xcd_block(int b, int nb) {
@@ -287,9 +301,11 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
 #pragma unroll
     for (int i = 0; i < TZ; ++i) {
       num[i] = 0.0f;
-      den[i] = 0.0f;
+      // 1e-30 is below half an ulp of any nonzero weight sum (W >= ~1e-13
+      // when a sample is selected), so den ends as the exact sum when a
+      // view contributes and 1e-30 (num = 0, no change to acc) otherwise.
+      den[i] = 1e-30f;
     }
-    int ilo_t = TZ, ihi_t = -1;
 
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
@@ -329,8 +345,6 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
           if (can_stop && (up ? ilo > nz - 1 : ihi < 0)) break;
           continue;
         }
-        ilo_t = min(ilo_t, ilo);
-        ihi_t = max(ihi_t, ihi);
         const float row0 = fmaf(kbs - ztv, invLh, g.cr);   // fct_bp_row0
         ChanInterp<PREC> ci;
         ci.init((uint32_t)(v - vr.x) * vstride * kEntry, pf, wp);
@@ -364,8 +378,7 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
     if (MODE == kBpAccumulate) {
 #pragma unroll
       for (int i = 0; i < TZ; ++i)
-        if (i >= ilo_t && i <= ihi_t)
-          acc[i] = fmaf(num[i], __builtin_amdgcn_rcpf(fmaxf(den[i], 1e-30f)), acc[i]);
+        acc[i] = fmaf(num[i], __builtin_amdgcn_rcpf(den[i]), acc[i]);
     } else if (MODE == kBpMatched) {
 #pragma unroll
       for (int i = 0; i < TZ; ++i) acc[i] += num[i];
