@@ -176,10 +176,18 @@ struct ChanInterp<FCT_PRECISION_BF16> {   // quad entries: one 8-byte load
 // offset and get weight 0.
 template <int PREC>
 struct BpTap {
-  float qi, qfl;
+  float qi, fq;   // row position and its fraction qi - floor(qi)
   bool sel;
   typename ChanInterp<PREC>::Raw raw;
 };
+
+// floor(x) as an int in one instruction (hipcc emits v_floor + v_cvt for
+// (int)floorf(x)); same value for every finite x in int range.
+__device__ __forceinline__ int floor_int(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
 
 template <int PREC, int MODE, bool CENTERED>
 __device__ __forceinline__ BpTap<PREC> bp_fetch(float zsi, float invLh, float row0,
@@ -188,13 +196,15 @@ __device__ __forceinline__ BpTap<PREC> bp_fetch(float zsi, float invLh, float ro
                                                 bool chan_ok, const FctBpConsts& g) {
   BpTap<PREC> s;
   s.qi = fmaf(zsi, invLh, row0);   // fct_bp_row
-  s.qfl = floorf(s.qi);
-  const int iq = (int)s.qfl;
+  const int iq = floor_int(s.qi);
+  // qi - floor(qi), exact for qi >= 0 (the selected rows); v_fract clamps
+  // below 1 where that difference would round up, which only happens for
+  // tiny negative qi (not selected).
+  s.fq = __builtin_amdgcn_fractf(s.qi);
   if (CENTERED)   // |q| <= 1 is implied by the row test when cr = hr
     s.sel = chan_ok && (uint32_t)iq <= (uint32_t)(g.nrows - 2);
   else
-    s.sel = chan_ok && !(fabsf(s.qi - g.cr) > g.hr) && s.qfl >= 0.0f &&
-            s.qfl <= (float)(g.nrows - 2);
+    s.sel = chan_ok && !(fabsf(s.qi - g.cr) > g.hr) && iq >= 0 && iq <= g.nrows - 2;
   // 24-bit multiply: exact for selected rows (< 2^24).
   if (MODE != kBpCount)
     s.raw = ChanInterp<PREC>::fetch(rf, ci.base + __umul24((uint32_t)iq, rowB), (int)rowB);
@@ -209,7 +219,7 @@ __device__ __forceinline__ void bp_add(const BpTap<PREC>& s, const ChanInterp<PR
     return;
   }
   const float2 lh = ci.rows(s.raw);
-  const float val = fmaf(s.qi - s.qfl, lh.y - lh.x, lh.x);
+  const float val = fmaf(s.fq, lh.y - lh.x, lh.x);
   if (MODE == kBpMatched) {   // plain sum over the views, weight R / l
     num = fmaf(s.sel ? fR : 0.0f, val, num);
   } else {

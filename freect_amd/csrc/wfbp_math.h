@@ -163,19 +163,20 @@ FCT_HD bool fct_bp_select(float qi, float pidx, const FctBpConsts& g, int* qf, i
 
 // W(q) = 1 for |q| < Q, cos^2(pi/2 (|q|-Q)/(1-Q)) for Q <= |q| < 1, written
 // as sin^2(pi/2 t) with t = min((1-|q|)/(1-Q), 1) for selected samples
-// (|q| <= 1).  sin(pi/2 t) = t * S(t^2) with S a degree-4 polynomial fitted
-// for minimax relative error of sin(pi/2 t) / t on [0, 1] under the
-// constraint S(1) = 1 (Lawson-weighted least squares): FP32 relative error
-// of W below 4.2e-7 on [0, 1], including near W -> 0 where an
-// absolute-accuracy cosine would not be, and W = 1 exactly in the flat part
-// (t = 1).  Not part of index selection.
+// (|q| <= 1).  sin(pi/2 t) = t * S(t^2) with S a cubic fitted for minimax
+// relative error of sin(pi/2 t) / t on [0, 1] under the constraint S(1) = 1
+// (Lawson-weighted least squares; the constant term then moved by one ulp so
+// that the FP32 evaluation gives S(1) = 1): FP32 relative error of W below
+// 2.5e-6 on [0, 1], including near W -> 0 where an absolute-accuracy cosine
+// would not be, and W = 1 exactly in the flat part (t = 1).  W only weights
+// the views of a theta~ against each other; it is not part of index
+// selection.
 FCT_HD float fct_w_of_t(float t) {
   const float u = t * t;
-  float s = 0.00015128079f;
-  s = fmaf(s, u, -0.0046733934f);
-  s = fmaf(s, u, 0.079689465f);
-  s = fmaf(s, u, -0.64596367f);
-  s = fmaf(s, u, 1.5707964f);
+  float s = -0.0043512345f;
+  s = fmaf(s, u, 0.07947272f);
+  s = fmaf(s, u, -0.6459161f);
+  s = fmaf(s, u, 1.5707947f);
   return u * (s * s);
 }
 
