@@ -7,9 +7,10 @@ into 512 x 512 x 320 slices at 0.625 mm, 500 mm FOV, ramp filter.
 
 A step is one whole reconstruction (rebin + filter -> backprojection ->
 thickening) with the raw scan already resident in HBM.  The chosen path
-stores filtered projections as FP16 with a per-(view,row) FP32 scale,
-accumulates in FP32 and runs as a Batch: the step recorded once into a
-hipGraph, the persistent backprojection, the tile depth from the autotuner.
+stores filtered projections as FP16 2 x 2 bilinear quads with one FP32 scale
+per scan, accumulates in FP32 and runs as a Batch: the step recorded once
+into a hipGraph, the persistent backprojection, the tile depth from the
+autotuner.
 The plain FP32 single-plan path (and the FP16 single-plan path) are timed
 beside it.  value is voxel-views/s of the chosen path, where a voxel-view is
 a (native voxel, view) pair selected by the reconstruction rule (counted

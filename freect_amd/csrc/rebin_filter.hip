@@ -132,6 +132,32 @@ __device__ __forceinline__ void quad_store_lines(const RebinFilterArgs& a, size_
   if (has1) quad_store<T>(q, o1 + c, r1, c, a.nrows, a.cpitch, to_store<T>(y1));
 }
 
+template <class T>
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+  const T l = to_store<T>(lo), h = to_store<T>(hi);
+  return (uint32_t)__builtin_bit_cast(uint16_t, l) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
+}
+
+// Rows r (even) and r + 1 of one view at channel c, entry index e of (r, c),
+// with the values at c and c + 1 of both rows: the whole quad (r, c), the
+// first half of (r + 1, c) (with a zero second half on the last row) and the
+// second half of (r - 1, c).  Together with the other pairs of the view every
+// slot is written exactly once.
+template <class T>
+__device__ __forceinline__ void quad_store_pair(const RebinFilterArgs& a, size_t e, int r,
+                                                float y00, float y01, float y10, float y11) {
+  uint32_t* q = static_cast<uint32_t*>(a.out);   // two 32-bit halves per entry
+  const uint32_t lo = pack2<T>(y00, y01), hi = pack2<T>(y10, y11);
+  *reinterpret_cast<uint2*>(q + e * 2) = make_uint2(lo, hi);
+  const size_t e1 = e + (size_t)a.cpitch;
+  if (r + 1 == a.nrows - 1)
+    *reinterpret_cast<uint2*>(q + e1 * 2) = make_uint2(hi, 0u);
+  else
+    q[e1 * 2] = hi;
+  if (r > 0) q[(e - (size_t)a.cpitch) * 2 + 1] = lo;
+}
+
 // max|x| over the raw scan as FP32 bits in scale[1] (non-negative floats
 // order like their bit patterns).
 __global__ void scale_reset_kernel(float* scale) {
@@ -290,6 +316,7 @@ __global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs
   constexpr bool kPingPong = N <= 2048;
   constexpr int kSlots = N + N / 8;
   __shared__ FctCplx lds[kPingPong ? 2 * kSlots : kSlots];
+  __shared__ float edge[T / 64][4][2];   // lane 0 of each wave: FP16 / BF16 neighbours
   const int t = threadIdx.x;
   const FctCplx* tw = reinterpret_cast<const FctCplx*>(a.tw);
 
@@ -372,17 +399,59 @@ __global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs
       }
       continue;
     }
+    float ya[4], yb[4];   // the pair's scaled values at channel t + m T (0 past ncols)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const FctCplx y = w[fct_fft_final_reg<N>(m)];
+      const bool in = t + m * T < a.ncols;
+      ya[m] = in ? y.x * is : 0.0f;
+      yb[m] = in ? -y.y * is : 0.0f;
+    }
+    if (has1 && (r0 & 1) == 0 && r1 == r0 + 1) {
+      // Rows r0 (even) and r0 + 1 of one view: the 2 x 2 quads of row r0 are
+      // complete here (one 8-byte store), row r0 + 1 gives the first half of
+      // its quads (the second comes from the next pair, or is 0 on the last
+      // row) and the second half of row r0 - 1's.  Channel c + 1 is held by
+      // lane t + 1 (same m), or across a wave boundary by lane 0 of the next
+      // wave, or for t = T - 1 by thread 0 at m + 1 (0 past N / 2 >= cpitch).
+      if ((t & 63) == 0) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          edge[t >> 6][m][0] = ya[m];
+          edge[t >> 6][m][1] = yb[m];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int c = t + m * T;
+        float na = __shfl_down(ya[m], 1, 64), nb = __shfl_down(yb[m], 1, 64);
+        if ((t & 63) == 63) {
+          const int wn = (t >> 6) + 1;
+          if (wn < T / 64) {
+            na = edge[wn][m][0];
+            nb = edge[wn][m][1];
+          } else {
+            na = m < 3 ? edge[0][m + 1][0] : 0.0f;
+            nb = m < 3 ? edge[0][m + 1][1] : 0.0f;
+          }
+        }
+        if (c >= a.cpitch) continue;
+        if (a.precision == FCT_PRECISION_FP16)
+          quad_store_pair<__half>(a, o0 + c, r0, ya[m], na, yb[m], nb);
+        else
+          quad_store_pair<__hip_bfloat16>(a, o0 + c, r0, ya[m], na, yb[m], nb);
+      }
+      continue;
+    }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const int c = t + m * T;
       if (c >= a.cpitch) continue;
-      const FctCplx y = w[fct_fft_final_reg<N>(m)];
-      const bool in = c < a.ncols;
-      const float y0 = in ? y.x * is : 0.0f, y1 = in ? -y.y * is : 0.0f;
       if (a.precision == FCT_PRECISION_FP16)
-        quad_store_lines<__half>(a, o0, o1, r0, r1, has1, c, y0, y1);
+        quad_store_lines<__half>(a, o0, o1, r0, r1, has1, c, ya[m], yb[m]);
       else
-        quad_store_lines<__hip_bfloat16>(a, o0, o1, r0, r1, has1, c, y0, y1);
+        quad_store_lines<__hip_bfloat16>(a, o0, o1, r0, r1, has1, c, ya[m], yb[m]);
     }
   }
 }
