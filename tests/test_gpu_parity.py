@@ -137,6 +137,64 @@ def test_rebin_filter_fft_sizes(ctx, oracle, n_channels):
         assert np.all(q[:, -1, :, 3] == 0) and np.all(q[:, :, -1, 3] == 0)
 
 
+@pytest.mark.parametrize("n_channels,n_rows", [(150, 5), (96, 5), (150, 6)])
+def test_fp16_quads_every_slot(ctx, oracle, n_channels, n_rows):
+    # The FP16 quad writer on both kernels (nfft 512: register kernel, which
+    # stores row pairs whole when a line pair is rows 2i, 2i+1 of one view and
+    # falls back to 16-bit slot stores when an odd row count makes pairs
+    # straddle views; nfft 256: the LDS kernel), checked slot by slot against
+    # the stored (r, c) values, and the stored values against the FP32 rows.
+    w = S.tiny(n_channels=n_channels, n_rows=n_rows, ppr=96, rotations=2.0)
+    sc = w.scan
+    raw_t = _raw_gpu(ctx, w, 0.0)
+    say("fp16 quads", n_channels, "channels", n_rows, "rows")
+    p32 = _plan(ctx, w)
+    f32_t, _ = p32.alloc_filtered()
+    p32.rebin_filter(raw_t, f32_t)
+    p = _plan(ctx, w, precision="fp16")
+    filt_t, scale_t = p.alloc_filtered()
+    p.rebin_filter(raw_t, filt_t, scale_t)
+    p.synchronize()
+    f32 = to_np(p32.filtered_view(f32_t)).astype(np.float64)
+    q = to_np(p.filtered_quads(filt_t).float())
+    s = float(to_np(scale_t)[0])
+    y = q[..., 0]
+    assert np.array_equal(q[:, :, :-1, 1], y[:, :, 1:]) and np.all(q[:, :, -1, 1] == 0)
+    assert np.array_equal(q[:, :-1, :, 2], y[:, 1:, :]) and np.all(q[:, -1, :, 2] == 0)
+    assert np.array_equal(q[:, :-1, :-1, 3], y[:, 1:, 1:])
+    assert np.all(q[:, -1, :, 3] == 0) and np.all(q[:, :, -1, 3] == 0)
+    err = np.abs(y.astype(np.float64) * s - f32) - (2.0 ** -11 * np.abs(f32) + s * 2.0 ** -24)
+    assert np.all(err <= 1e-6 * np.abs(f32).max())
+
+
+def test_fp16_direct_variant_reads_quads(ctx, oracle):
+    # The direct kernel reads the same FP16 quads (one scale per scan) as the
+    # lean kernel: the two volumes agree to FP32 round-off, and both are within
+    # the mixed-precision bound of the FP64 restatement.
+    w = S.tiny(nx=48, n_slices=12, n_channels=192, n_rows=16, ppr=192, rotations=3)
+    sc, rc = w.scan, w.recon
+    d = oracle.geometry(sc, rc)
+    raw_t = _raw_gpu(ctx, w, 5e4)
+    ref, _ = oracle.reconstruct_native(sc, rc, to_np(raw_t), 0, d.n_slices_native)
+    vols = {}
+    for variant in (1, 2):
+        p = _plan(ctx, w, precision="fp16", bp_variant=variant)
+        filt_t, sc_t = p.alloc_filtered()
+        p.rebin_filter(raw_t, filt_t, sc_t)
+        nat = p.alloc_native()
+        p.backproject(filt_t, sc_t, nat)
+        p.synchronize()
+        vols[variant] = to_np(nat).astype(np.float64)
+    mask = fov_mask(rc, d.acq_fov)
+    diff = hu_errors(vols[1], vols[2], mask)
+    say("fp16 direct vs lean: rmse %.2e max %.2e HU" % diff)
+    assert diff[1] < 0.01
+    for variant in (1, 2):
+        rmse, mx = hu_errors(vols[variant], ref, mask)
+        say("fp16 variant %d vs fp64: rmse %.4f max %.4f HU" % (variant, rmse, mx))
+        assert rmse <= 0.5
+
+
 @pytest.mark.parametrize("cr_shift", [0.0, 0.37])
 def test_selection_bit_exact_counts(ctx, oracle, cr_shift):
     # cr_shift != 0 moves the central row off (nrows-1)/2 and exercises the
