@@ -69,10 +69,35 @@ def time_path(plan, raw, steps, warmup, torch):
     return (t1 - t0) * 1e3 / steps, bp_ms, vol
 
 
-def cpu_baseline(w, raw_np, n_vv_total, n_native):
-    """FP64 CPU restatement on a bounded sample, scaled to the whole job:
-    rebin + filter of 96 views (scaled to all views) + backprojection of one
-    native slice (scaled by the exact voxel-view count)."""
+def nominal_voxel_views(w, info):
+    """The closed form in-FOV voxels x native slices x ppr / pitch (every
+    in-FOV voxel seen for 1/pitch rotations); the exact count comes from the
+    GPU selection rule."""
+    import numpy as np
+    rc, sc = w.recon, w.scan
+    x = (np.arange(rc.nx) - 0.5 * (rc.nx - 1)) * rc.recon_fov / rc.nx + rc.x_origin
+    y = (np.arange(rc.ny) - 0.5 * (rc.ny - 1)) * rc.recon_fov / rc.ny + rc.y_origin
+    X, Y = np.meshgrid(x, y)
+    in_fov = int(np.count_nonzero(X * X + Y * Y <= (0.5 * info.acquisition_fov) ** 2))
+    pitch = abs(sc.feed) / (sc.n_rows * info.collimated_slice_width)
+    return float(in_fov * info.n_slices_native * sc.views_per_rotation / pitch)
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, raw_np, n_vv_total, n_native, n_slices=8, n_rf_views=960):
+    """FP64 CPU restatement on a bounded sample (about 10 s on 16 cores),
+    scaled to the whole job: rebin + filter of n_rf_views views (scaled to all
+    views) + backprojection of n_slices central native slices (scaled by the
+    exact voxel-view count)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
@@ -85,25 +110,29 @@ def cpu_baseline(w, raw_np, n_vv_total, n_native):
     except Exception:
         pass
     # (a) rebin + filter sample.
+    va = sc.n_views // 2 - n_rf_views // 2
     t = time.perf_counter()
-    par = pyoracle.rebin(sc, raw_np, 4000, 4096)
+    par = pyoracle.rebin(sc, raw_np, va, va + n_rf_views)
     h = pyoracle.filter_kernel(d.nfft, d.ds, rc.filter_kind, rc.filter_c, rc.filter_a)
     par = pyoracle.filter_rows(par, h, d.nfft, use_fft=True)
-    t_rf = (time.perf_counter() - t) * sc.n_views / 96.0
-    # (b) backprojection of one central native slice.
-    s0 = n_native // 2
-    v0, v1 = pyoracle.view_range(sc, rc, s0, s0 + 1)
+    t_rf_sample = time.perf_counter() - t
+    t_rf = t_rf_sample * sc.n_views / float(n_rf_views)
+    # (b) backprojection of central native slices.
+    s0 = n_native // 2 - n_slices // 2
+    v0, v1 = pyoracle.view_range(sc, rc, s0, s0 + n_slices)
     par = pyoracle.rebin(sc, raw_np, v0, v1)
     par = pyoracle.filter_rows(par, h, d.nfft, use_fft=True)
     t = time.perf_counter()
-    vol, n_sel = pyoracle.backproject(sc, rc, par, v0, s0, s0 + 1)
-    t_bp1 = time.perf_counter() - t
-    t_total = t_rf + t_bp1 * n_vv_total / max(n_sel, 1)
+    vol, n_sel = pyoracle.backproject(sc, rc, par, v0, s0, s0 + n_slices)
+    t_bp = time.perf_counter() - t
+    t_total = t_rf + t_bp * n_vv_total / max(n_sel, 1)
     return {"value": n_vv_total / t_total, "unit": "voxel-views/s", "cores": cores,
             "kind": "port",
-            "sample": ("FP64 OpenMP restatement: rebin+filter of 96 of 9840 views scaled x102.5, "
-                       f"plus backprojection of native slice {s0} ({n_sel} voxel-views) scaled by "
-                       "the exact voxel-view count"),
+            "sample": (f"FP64 OpenMP restatement on {cores} threads ({_cpu_model()}): rebin+filter "
+                       f"of {n_rf_views} of {sc.n_views} views ({t_rf_sample:.2f} s) scaled to all "
+                       f"views, plus backprojection of native slices {s0}..{s0 + n_slices - 1} "
+                       f"({n_sel} voxel-views, {t_bp:.2f} s) scaled by the exact voxel-view count"),
+            "stage_s": {"rebin_filter": t_rf, "backproject": t_bp * n_vv_total / max(n_sel, 1)},
             "s_per_volume_est": t_total}, (s0, vol)
 
 
@@ -456,16 +485,17 @@ def main():
         log("cpu baseline")
         raw_np = raw.cpu().numpy()
         cpu, (s0, ref) = cpu_baseline(w, raw_np, n_vv, info.n_slices_native)
-        # HU error of both GPU paths against the FP64 restatement on that slice
-        # (native == requested grid for this workload).
+        # HU error of the GPU paths against the FP64 restatement on those
+        # slices (native == requested grid for this workload).
         x = (np.arange(w.recon.nx) - 0.5 * (w.recon.nx - 1)) * w.recon.recon_fov / w.recon.nx
         X, Y = np.meshgrid(x, x)
         mask = X * X + Y * Y <= (0.5 * info.acquisition_fov) ** 2
         for prec, vol in vols.items():
-            d = (vol[s0].cpu().numpy().astype(np.float64) - ref[0]) / (S.MU_WATER / 1000.0)
-            d = d[mask]
+            n = ref.shape[0]
+            d = (vol[s0:s0 + n].cpu().numpy().astype(np.float64) - ref) / (S.MU_WATER / 1000.0)
+            d = d[:, mask]
             hu[prec] = {"rmse": float(np.sqrt(np.mean(d * d))), "max": float(np.abs(d).max()),
-                        "slice": int(s0)}
+                        "slices": [int(s0), int(s0 + n - 1)]}
         log("hu", hu)
 
     bpv = BYTES_PER_VV[args.precision]
@@ -497,7 +527,8 @@ def main():
         "data": "synthetic (seeded ellipsoid phantom, Poisson noise I0=5e4, generated on the GPU)",
         "config": {"workload": "full: 888ch x 64 rows, 9840 views (984/rot, pitch 0.984), "
                                "512x512x320 slices at 0.625 mm, FOV 500 mm, ramp",
-                   "voxel_views": n_vv, "path": args.path,
+                   "voxel_views": n_vv, "voxel_views_nominal": nominal_voxel_views(w, info),
+                   "path": args.path,
                    "tile_z": chosen.get("tile_z", info.tile_z), "autotune_ms": tuned},
         "s_per_volume": chosen["ms_per_step"] * 1e-3,
         "plain_fp32": {"ms_per_step": res["fp32"]["ms_per_step"],
