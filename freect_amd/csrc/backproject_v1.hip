@@ -414,8 +414,13 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   }
 }
 
+// Seven waves per SIMD for the 16-bit kernels at TZ = 8 (the allocator would
+// take the registers of six; the few spilled bytes cost less than the lost
+// wave -- measured 1-2 % faster).
 template <int TZ, int PREC, int MODE, bool CENTERED>
-__global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(TZ == 8 && PREC != FCT_PRECISION_FP32 ? 7 : 1, 8)))
+void bp_v1_kernel(BpArgs a) {
   bp_v1_tile<TZ, PREC, MODE, CENTERED>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 
@@ -428,12 +433,12 @@ __global__ __launch_bounds__(64) void bp_v1_kernel(BpArgs a) {
 // eight counters have passed their range ends.  A tile is computed by the
 // same code as in bp_v1_kernel, so each job's volume is bitwise identical to
 // its own single-job launch with the same tile_z.
-// (Six waves per SIMD for the 16-bit kernels at TZ <= 8: the persistent loop
-// would otherwise hold the registers of five, and these fit six without
-// spilling.)
+// (Seven waves per SIMD for the 16-bit kernels at TZ <= 8, as for
+// bp_v1_kernel: the persistent loop would otherwise hold the registers of
+// five.)
 template <int TZ, int PREC, bool CENTERED>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(TZ <= 8 && PREC != FCT_PRECISION_FP32 ? 6 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(TZ <= 8 && PREC != FCT_PRECISION_FP32 ? 7 : 1, 8)))
 void bp_batch_kernel(BpBatchArgs b,
                                                       const BpArgs* __restrict__ jobs,
                                                       const int32_t* __restrict__ tile_start) {
