@@ -167,6 +167,36 @@ def test_fp16_quads_every_slot(ctx, oracle, n_channels, n_rows):
     assert np.all(err <= 1e-6 * np.abs(f32).max())
 
 
+@pytest.mark.parametrize("factor", [0.0, 1e-6, 1e6])
+def test_fp16_storage_scale_extremes(ctx, oracle, factor):
+    # The per-scan storage scale follows the data: a scan multiplied by 1e6
+    # (FP16 would overflow without it) or by 1e-6 (subnormal without it)
+    # reconstructs to the FP32 volume times the same factor within the
+    # mixed-precision error, and an all-zero scan gives scale 1 and zeros.
+    w = S.tiny(nx=32, n_slices=8, n_channels=96, n_rows=8, ppr=96, rotations=2.0)
+    raw_t = _raw_gpu(ctx, w, 0.0)
+    say("fp16 scale factor", factor)
+    vols = {}
+    for prec in ("fp32", "fp16"):
+        p = _plan(ctx, w, precision=prec)
+        filt_t, sc_t = p.alloc_filtered()
+        p.rebin_filter(raw_t * factor, filt_t, sc_t)
+        nat = p.alloc_native()
+        p.backproject(filt_t, sc_t, nat)
+        p.synchronize()
+        vols[prec] = to_np(nat).astype(np.float64)
+        if prec == "fp16":
+            s = float(to_np(sc_t)[0])
+            assert np.isfinite(vols[prec]).all()
+            if factor == 0.0:
+                assert s == 1.0 and not vols[prec].any()
+    if factor:
+        ref = vols["fp32"]
+        err = np.abs(vols["fp16"] - ref).max() / np.abs(ref).max()
+        say("  rel max err %.2e (scale %g)" % (err, s))
+        assert err < 2e-3
+
+
 def test_fp16_direct_variant_reads_quads(ctx, oracle):
     # The direct kernel reads the same FP16 quads (one scale per scan) as the
     # lean kernel: the two volumes agree to FP32 round-off, and both are within
