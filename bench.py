@@ -9,8 +9,9 @@ A step is one whole reconstruction (rebin + filter -> backprojection ->
 thickening) with the raw scan already resident in HBM.  The chosen path
 stores filtered projections as FP16 2 x 2 bilinear quads with one FP32 scale
 per scan, accumulates in FP32 and runs as a Batch: the step recorded once
-into a hipGraph, the persistent backprojection, the tile depth from the
-autotuner.
+into a hipGraph, the tile depth from the autotuner (a batch of this one job
+backprojects with the single-plan grid kernel; several jobs share one
+persistent launch).
 The plain FP32 single-plan path (and the FP16 single-plan path) are timed
 beside it.  value is voxel-views/s of the chosen path, where a voxel-view is
 a (native voxel, view) pair selected by the reconstruction rule (counted
@@ -137,12 +138,12 @@ def cpu_baseline(w, raw_np, n_vv_total, n_native, n_slices=8, n_rf_views=960):
 
 
 def batch_path(ctx, jobs, raws, precision, steps, warmup, torch, candidates=(4, 8, 16)):
-    """Chosen path: one Batch (per-job rebin+filter, one persistent
-    backprojection over every job's tiles, per-job thicken) recorded into a
-    hipGraph and replayed once per step, tile depth from the autotuner.
-    Returns ms/step, the persistent backprojection's mean ms (HIP events
-    around BP-only launches on the batch stream), the tuner's table and the
-    volumes."""
+    """Chosen path: one Batch (per-job rebin+filter, one backprojection
+    launch over every job's tiles -- persistent for several jobs, the grid
+    kernel for one -- per-job thicken) recorded into a hipGraph and replayed
+    once per step, tile depth from the autotuner.  Returns ms/step, the
+    backprojection's mean ms (HIP events around BP-only launches on the batch
+    stream), the tuner's table and the volumes."""
     from freect_amd import wfbp
     stream = torch.cuda.Stream()
     b = wfbp.Batch(ctx, jobs, precision=precision, stream=stream)
@@ -166,7 +167,7 @@ def batch_path(ctx, jobs, raws, precision, steps, warmup, torch, candidates=(4, 
     t1 = time.perf_counter()
     ms = ev[0].elapsed_time(ev[1]) / steps
     wall_ms = (t1 - t0) * 1e3 / steps
-    # The persistent backprojection alone, same stream, same buffers.
+    # The backprojection alone, same stream, same buffers.
     bev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
     for a, c in bev:
@@ -475,7 +476,7 @@ def main():
         vols["batch"] = bvols[0]
         log("batch", res["batch"])
         chosen = res["batch"]
-        kernel = "bp_batch_kernel"
+        kernel = "bp_v1_kernel"   # a batch of one job launches the grid kernel
     else:
         chosen = res[args.precision]
     info = p0.info
@@ -504,8 +505,7 @@ def main():
     # FETCH_SIZE / WRITE_SIZE passes committed under profiles/.
     traffic = None
     tp = os.path.join(ROOT, "profiles", "backproject_full_traffic.json")
-    tkey = args.precision if (args.path == "batch" or args.precision == "fp32") \
-        else args.precision + "_plan"
+    tkey = args.precision if args.precision == "fp32" else args.precision + "_plan"
     if os.path.exists(tp):
         try:
             traffic = json.load(open(tp)).get(tkey, {}).get("traffic_bytes")

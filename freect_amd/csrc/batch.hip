@@ -105,8 +105,21 @@ bool all_centered(const fct_wfbp_gpu_batch* b) {
 }
 
 // The persistent backprojection of every job (queue counters cleared by a
-// kernel, so the sequence can be recorded into a graph).
+// kernel, so the sequence can be recorded into a graph).  A batch of one job
+// has no tiles of other jobs to balance: it launches the single-plan grid
+// kernel, which runs the same tile code (the volume is bitwise the same) and
+// has no queue to walk.
 int enqueue_backproject(fct_wfbp_gpu_batch* b) {
+  if (b->plans.size() == 1) {
+    fct_wfbp_gpu_plan* p = b->plans[0];
+    fct::BpArgs a = fct::plan_bp_args(p);
+    a.filt = p->d_filtered;
+    a.filt_scale = p->d_scale;
+    a.native = p->d_native;
+    BATCH_TRY(b, fct::launch_backproject_v1(a, p->tile_z, b->opts.precision, fct::kBpAccumulate,
+                                            b->stream));
+    return FCT_OK;
+  }
   BATCH_TRY(b, fct::launch_fill(reinterpret_cast<float*>(b->d_queue), 8, 0.0f, b->stream));
   fct::BpBatchArgs a{};
   a.jobs = b->d_jobs;

@@ -258,7 +258,8 @@ FCT_API int fct_wfbp_gpu_phantom_project(fct_wfbp_gpu_ctx* ctx, const fct_scan_g
  * precision and tile_z, any geometry / matrix / slice range / filter) plus
  * every intermediate buffer.  One run is: rebin_filter of every job, ONE
  * persistent backprojection launch over the tiles of all jobs (work queue per
- * XCD), thicken of every job -- recorded once into a hipGraph and replayed.
+ * XCD; a batch of one job launches the single-plan grid kernel instead),
+ * thicken of every job -- recorded once into a hipGraph and replayed.
  * Each job's volume is bitwise identical to fct_wfbp_gpu_reconstruct on a
  * plan with the same options. */
 typedef struct fct_wfbp_gpu_batch fct_wfbp_gpu_batch;
@@ -292,11 +293,11 @@ FCT_API int fct_wfbp_gpu_batch_submit(fct_wfbp_gpu_batch* batch, int32_t job, co
 /* Enqueue one run on the batch stream (asynchronous). */
 FCT_API int fct_wfbp_gpu_batch_run(fct_wfbp_gpu_batch* batch, int32_t mode);
 FCT_API int fct_wfbp_gpu_batch_synchronize(fct_wfbp_gpu_batch* batch);
-/* Enqueue only the persistent backprojection of every job (eager), reading
+/* Enqueue only the backprojection of every job (eager), reading
  * the filtered projections the last run left in the batch's buffers and
  * rewriting the native volumes.  For timing the kernel on the batch stream. */
 FCT_API int fct_wfbp_gpu_batch_backproject(fct_wfbp_gpu_batch* batch);
-/* Autotuner: runs rebin_filter of every job once, then times the persistent
+/* Autotuner: runs rebin_filter of every job once, then times the
  * backprojection `reps` times for each candidate tile_z (HIP events on the
  * batch stream), keeps the fastest and returns each candidate's mean ms in
  * ms_out (host, n entries, may be NULL).  Synchronous; needs every job bound. */

@@ -1,5 +1,6 @@
 """Batched uneven reconstructions (fct_wfbp_gpu_batch_*): one persistent
-backprojection launch over every job's tiles, recorded into a hipGraph.
+backprojection launch over every job's tiles (the grid kernel for a batch of
+one job), recorded into a hipGraph.
 
 Pin: each job's volume is bitwise identical to a single-plan reconstruction
 with the same precision and tile depth (a tile is computed by the same code
@@ -85,6 +86,37 @@ def test_batch_equals_single_plans(ctx, precision):
     b.synchronize()
     for v, ref in zip(vols, refs):
         assert torch.equal(v, ref)
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_single_job_batch_equals_single_plan(ctx, precision):
+    # A batch of one job backprojects with the single-plan grid kernel: the
+    # same bitwise volume, eagerly and over graph replays, before and after
+    # the autotuner.
+    from freect_amd import wfbp
+    w = _jobs()[2]
+    raw = ctx.phantom_project(w.scan, w.phantom, 0.0, w.seed)
+    torch.cuda.synchronize()
+    b = wfbp.Batch(ctx, [(w.scan, w.recon)], precision=precision, tile_z=8)
+    vol = b.plans[0].alloc_volume()
+    b.submit(0, raw, vol)
+    ref = _single(ctx, w, raw, precision, 8)
+    for mode in (wfbp.Batch.EAGER, wfbp.Batch.GRAPH, wfbp.Batch.GRAPH):
+        vol.fill_(float("nan"))
+        torch.cuda.synchronize()
+        b.run(mode)
+        b.synchronize()
+        assert torch.equal(vol, ref)
+        torch.randn(1 << 20, device="cuda").sum().item()
+    tuned = b.autotune((4, 8), reps=1)
+    tz = b.info()["tile_z"]
+    say("single-job batch", precision, tuned, "->", tz)
+    ref = _single(ctx, w, raw, precision, tz)
+    vol.zero_()
+    b.run(wfbp.Batch.GRAPH)
+    b.synchronize()
+    assert torch.equal(vol, ref)
     b.close()
 
 
