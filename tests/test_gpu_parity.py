@@ -307,7 +307,8 @@ def test_thicken_matches_oracle(ctx, oracle):
 
 
 def test_small_workload_end_to_end(ctx, oracle):
-    """736x16, 2304 views, 256^2 x 32 slices at 1.25 mm, Shepp-Logan."""
+    """The small GE-like scan: 736x16, 2304 views, 256^2 x 32 slices at 1.25 mm,
+    Shepp-Logan."""
     w = S.small()
     sc, rc = w.scan, w.recon
     d = oracle.geometry(sc, rc)
