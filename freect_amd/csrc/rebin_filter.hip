@@ -36,10 +36,13 @@ __device__ __forceinline__ float rebin_sample(const float* __restrict__ raw, int
   // Bilinear at (projection j + aoff + fa of the entry's focal state, fan
   // channel b0 + fb) with the projection clamped to the scan, as the
   // reference's clamp-addressed texture; without FFS the state is 0 of 1 and
-  // the projection is the fan view.
-  const int rho = e.state - 1;
-  const int ma = min(max(j + e.aoff, 0), e.mmax);
-  const int mb = min(max(j + e.aoff + 1, 0), e.mmax);
+  // the projection is the fan view.  Every address stays inside the scan
+  // also for an entry without a fan sample (state 0: channels 0, offset 0,
+  // the state's mmax >= 0), read as focal state 0 -- the caller discards
+  // that value.
+  const int rho = max(e.state - 1, 0);
+  const int ma = max(min(j + e.aoff, e.mmax), 0);
+  const int mb = max(min(j + e.aoff + 1, e.mmax), 0);
   const int row = q >> a.zshift;
   const float* ra = raw + ((size_t)(ma * a.states + rho) * a.raw_nrows + row) * a.raw_ncols;
   const float* rb = raw + ((size_t)(mb * a.states + rho) * a.raw_nrows + row) * a.raw_ncols;
@@ -54,6 +57,20 @@ __device__ __forceinline__ float rebin_sample(const float* __restrict__ raw, int
 // with z FFS).
 __device__ __forceinline__ const RebinDev& rebin_entry_of(const RebinFilterArgs& a, int q, int c) {
   return a.tab[(size_t)(q & a.zshift) * a.ncols + c];
+}
+
+// Rebinned samples of line (parallel view j, row q) at channels t + m T,
+// m = 0..3 (the register kernel's ownership).
+__device__ __forceinline__ void gather_line(const RebinFilterArgs& a, int t, int T, int j, int q,
+                                            float out[4]) {
+  RebinDev e[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) e[m] = rebin_entry_of(a, q, min(t + m * T, a.ncols - 1));
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const float v = rebin_sample(a.raw, j, q, e[m], a);
+    out[m] = t + m * T < a.ncols && e[m].state ? v : 0.0f;
+  }
 }
 
 __device__ FctCplx* fft_forward(FctCplx* a, FctCplx* b, const FctCplx* __restrict__ tw, int N,
@@ -310,7 +327,8 @@ __device__ __forceinline__ void fft_forward_regs(FctCplx* v, FctCplx* lds, int t
 }
 
 template <int N>
-__global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs a) {
+__global__ __launch_bounds__(N / 8)
+void rebin_filter_reg_kernel(RebinFilterArgs a) {
   using S = FctFftShape<N>;
   constexpr int T = S::T;
   constexpr bool kPingPong = N <= 2048;
@@ -341,26 +359,35 @@ __global__ __launch_bounds__(N / 8) void rebin_filter_reg_kernel(RebinFilterArgs
     const int j0 = l0 / a.nrows, r0 = l0 - j0 * a.nrows;
     const int j1 = has1 ? l1 / a.nrows : 0, r1 = has1 ? l1 - j1 * a.nrows : 0;
     FctCplx v[8];
+    if (a.lines_in) {
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int c = t + m * T;
-      float va = 0.0f, vb = 0.0f;
-      if (a.lines_in) {
+      for (int m = 0; m < 4; ++m) {
+        const int c = t + m * T;
+        float va = 0.0f, vb = 0.0f;
         if (c < a.ncols) {
           va = a.lines_in[(size_t)l0 * a.cpitch + c];
           if (has1) vb = a.lines_in[(size_t)l1 * a.cpitch + c];
         }
-      } else if (c < a.ncols) {
-        // Small tables, cache resident.  No fan sample: 0 (cuda_kernels.cuh:32-34).
-        const RebinDev e0 = rebin_entry_of(a, r0, c);
-        if (e0.state) va = rebin_sample(a.raw, j0, r0, e0, a);
-        if (has1) {
-          const RebinDev e1 = rebin_entry_of(a, r1, c);
-          if (e1.state) vb = rebin_sample(a.raw, j1, r1, e1, a);
-        }
+        v[m] = FctCplx{va, vb};
+        v[m + 4] = FctCplx{0.0f, 0.0f};
       }
-      v[m] = FctCplx{va, vb};
-      v[m + 4] = FctCplx{0.0f, 0.0f};
+    } else {
+      // One line at a time, the table entries of the thread's four channels
+      // and then all their taps in flight together (no branch between them;
+      // rebin_sample stays inside the scan for every entry).  Channels past
+      // ncols read the last channel's entry; those and the entries without a
+      // fan sample give 0 (cuda_kernels.cuh:32-34).
+      float va[4], vb[4];
+      gather_line(a, t, T, j0, r0, va);
+      if (has1)
+        gather_line(a, t, T, j1, r1, vb);
+      else
+        vb[0] = vb[1] = vb[2] = vb[3] = 0.0f;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        v[m] = FctCplx{va[m], vb[m]};
+        v[m + 4] = FctCplx{0.0f, 0.0f};
+      }
     }
     if (a.rebin_only) {   // rebinned rows, FP32, no filter (no LDS exchange: the
                           // exchange parity of the next pair is unchanged)
