@@ -190,11 +190,16 @@ __global__ __launch_bounds__(256) void absmax_kernel(const float* __restrict__ x
   if (kVec) {
     const float4* x4 = reinterpret_cast<const float4*>(x);
     const int64_t n4 = n >> 2;
-    for (; i < n4; i += stride) {
-      const float4 v = x4[i];
-      m = max(m, max(max(__float_as_uint(fabsf(v.x)), __float_as_uint(fabsf(v.y))),
-                     max(__float_as_uint(fabsf(v.z)), __float_as_uint(fabsf(v.w)))));
+    auto absmax4 = [](const float4 v) {
+      return max(max(__float_as_uint(fabsf(v.x)), __float_as_uint(fabsf(v.y))),
+                 max(__float_as_uint(fabsf(v.z)), __float_as_uint(fabsf(v.w))));
+    };
+    for (; i + 3 * stride < n4; i += 4 * stride) {   // four loads in flight per thread
+      const float4 v0 = x4[i], v1 = x4[i + stride], v2 = x4[i + 2 * stride],
+                   v3 = x4[i + 3 * stride];
+      m = max(max(m, max(absmax4(v0), absmax4(v1))), max(absmax4(v2), absmax4(v3)));
     }
+    for (; i < n4; i += stride) m = max(m, absmax4(x4[i]));
     i = 4 * n4 + (int64_t)blockIdx.x * 256 + threadIdx.x;
   }
   for (; i < n; i += stride) m = max(m, __float_as_uint(fabsf(x[i])));
