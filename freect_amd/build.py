@@ -35,9 +35,19 @@ def hipcc() -> str:
     return p
 
 
+# Per-source additions.  backproject_v1.hip: the per-slice accumulators
+# (num[TZ], den[TZ]) stay separate registers.  Promoted to one <TZ x float>
+# vector, every view iteration of the backprojection copied both tuples
+# (16 v_mov_b64 at TZ = 8) and the FP16 kernel needed 72 instead of 63 VGPRs.
+EXTRA_FLAGS = {
+    "backproject_v1.hip": ["-mllvm", "-amdgpu-promote-alloca-to-vector-limit=1"],
+}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *HIPCC_FLAGS, *EXTRA_FLAGS.get(src, []), "-c", os.path.join(CSRC, src),
+           "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

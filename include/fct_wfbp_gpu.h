@@ -36,8 +36,8 @@ typedef enum {
  * backproject.  Accumulation is FP32 in every mode. */
 typedef enum {
   FCT_PRECISION_FP32 = 0,
-  FCT_PRECISION_FP16 = 1,   /* FP16 values x per-(view,row) FP32 scale */
-  FCT_PRECISION_BF16 = 2    /* BF16 values x per-(view,row) FP32 scale */
+  FCT_PRECISION_FP16 = 1,   /* FP16 quads x one power-of-two FP32 scale per scan */
+  FCT_PRECISION_BF16 = 2    /* BF16 quads x one power-of-two FP32 scale per scan */
 } fct_precision;
 
 /* Scan description: what RawDataSet.h:91-121 exposes, with the per-view tube
