@@ -263,7 +263,7 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   // Slice spacing inside the slab (uniform, FP32; used only for skipping).
   const float zfirst = zs[0], zlast = zs[nz - 1];
   const float dzs = nz > 1 ? (zlast - zfirst) / (float)(nz - 1) : 1.0f;
-  const float inv_dzs = 1.0f / dzs;
+  const float inv_dzs = 1.0f / dzs, ninv_dzs = -inv_dzs;
   // Along k (views t + k H of one theta~ and parity) the view z moves
   // monotonically, and with it every bound of a view's slice range [ilo, ihi]
   // below, all in the same direction: up (in slice index) when the table feed
@@ -335,22 +335,24 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
       // z-interval of |q| <= 1 relative to the view's table z, over the wave.
       const float A = wave_min(active ? -kbs - lt : BIG);
       const float B = wave_max(active ? -kbs + lt : -BIG);
+      // Slice positions of the interval ends for a view at table z ztv:
+      // ztv * inv_dzs + c, the lower end first (A <= B, so the order is
+      // that of the slice step's sign).
+      const float ca = (A - zfirst) * inv_dzs, cb = (B - zfirst) * inv_dzs;
+      const float clo = inv_dzs > 0.0f ? ca : cb, nchi = -(inv_dzs > 0.0f ? cb : ca);
       for (; k < kmax; k += 2) {
         const int v = t + k * H;
         const float ztv = ld_uniform(ztc + v);
         // Exact window slice range (native indices).
         const int2 wr = make_int2(ld_uniform(wrc + 2 * v), ld_uniform(wrc + 2 * v + 1));
-        float fa = (ztv + A - zfirst) * inv_dzs, fbb = (ztv + B - zfirst) * inv_dzs;
-        if (fa > fbb) {
-          const float tmp = fa;
-          fa = fbb;
-          fbb = tmp;
-        }
+        // One slice of margin on each side covers the rounding of these
+        // positions; the per-sample test below is exact.
+        const int flo = floor_int(fmaf(ztv, inv_dzs, clo));
+        const int fhi = floor_int(fmaf(ztv, ninv_dzs, nchi));   // -ceil(upper end)
         // (The matched projector pair uses every view that sees the voxel:
         // no view window.)
-        const int ilo = max(max(0, (int)floorf(fa) - 1), MODE == kBpMatched ? 0 : wr.x - s0);
-        const int ihi = min(min(nz - 1, (int)ceilf(fbb) + 1),
-                            MODE == kBpMatched ? nz - 1 : wr.y - s0);
+        const int ilo = max(max(0, flo - 1), MODE == kBpMatched ? 0 : wr.x - s0);
+        const int ihi = min(min(nz - 1, 1 - fhi), MODE == kBpMatched ? nz - 1 : wr.y - s0);
         if (ilo > ihi) {
           if (can_stop && (up ? ilo > nz - 1 : ihi < 0)) break;
           continue;
@@ -414,12 +416,17 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   }
 }
 
-// Seven waves per SIMD for the 16-bit kernels at TZ = 8 (the allocator would
-// take the registers of six; the few spilled bytes cost less than the lost
-// wave -- measured 1-2 % faster).
+// Occupancy floors for the 16-bit kernels: at least seven waves per SIMD at
+// TZ = 8 and six at TZ = 16 (the registers of five otherwise; the spills this
+// costs sit in the per-theta~ setup, outside the sample loop).
+template <int TZ, int PREC>
+constexpr int bp_min_waves() {
+  return PREC == FCT_PRECISION_FP32 ? 1 : TZ <= 8 ? 7 : TZ == 16 ? 6 : 1;
+}
+
 template <int TZ, int PREC, int MODE, bool CENTERED>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(TZ == 8 && PREC != FCT_PRECISION_FP32 ? 7 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(TZ == 8 || TZ == 16 ? bp_min_waves<TZ, PREC>() : 1, 8)))
 void bp_v1_kernel(BpArgs a) {
   bp_v1_tile<TZ, PREC, MODE, CENTERED>(a, xcd_block(blockIdx.x, gridDim.x));
 }
@@ -433,12 +440,10 @@ void bp_v1_kernel(BpArgs a) {
 // eight counters have passed their range ends.  A tile is computed by the
 // same code as in bp_v1_kernel, so each job's volume is bitwise identical to
 // its own single-job launch with the same tile_z.
-// (Seven waves per SIMD for the 16-bit kernels at TZ <= 8, as for
-// bp_v1_kernel: the persistent loop would otherwise hold the registers of
-// five.)
+// (The occupancy floors of bp_v1_kernel, also at TZ = 4.)
 template <int TZ, int PREC, bool CENTERED>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(TZ <= 8 && PREC != FCT_PRECISION_FP32 ? 7 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(bp_min_waves<TZ, PREC>(), 8)))
 void bp_batch_kernel(BpBatchArgs b,
                                                       const BpArgs* __restrict__ jobs,
                                                       const int32_t* __restrict__ tile_start) {
