@@ -134,7 +134,8 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
       const float pfl = floorf(pidx);
       const bool chan_ok = active && pfl >= 0.0f && pfl <= (float)(g.ncols - 2);
       const int pf = chan_ok ? (int)pfl : 0;
-      const float wp = pidx - pfl;
+      // (FP16: the channel weight of the lean kernel's FP16 interpolation.)
+      const float wp = PREC == FCT_PRECISION_FP16 ? fp16_channel_weight(pidx - pfl) : pidx - pfl;
       const T* vbase = filt + (size_t)v * vstride + pf;   // FP32 rows
       const T* qbase = filt + ((size_t)v * vstride + pf) * 4;   // FP16 / BF16 quads
 #pragma unroll

@@ -20,10 +20,11 @@
 //     test, one unsigned compare;
 //   * taps: FP32 rows are two 8-byte buffer loads (the second row through the
 //     scalar offset); FP16 / BF16 projections are stored as 2 x 2 quads, one
-//     8-byte load per sample, interpolated across channels straight from the
-//     FP16 halves (v_fma_mix_f32), and scaled once per scan at the end; lanes that
-//     select nothing read whatever the range-checked buffer returns and get
-//     weight 0 instead of a branch;
+//     8-byte load per sample, and scaled once per scan at the end; FP16 quads
+//     are interpolated across channels by two v_dot2_f32_f16 with the channel
+//     weights packed as FP16 (exact products, FP32 sums); lanes that select
+//     nothing read whatever the range-checked buffer returns and get weight 0
+//     instead of a branch;
 //   * the per-theta~ mean uses v_rcp.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -81,7 +82,6 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * q + min(x, r) + i;
 }
 
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 // Read-only table entry at a wave-uniform address, loaded through the
 // constant address space so it becomes a scalar load into SGPRs even where
@@ -91,9 +91,10 @@ __device__ __forceinline__ T ld_uniform(const T* p) {
   return *(const __attribute__((address_space(4))) T*)p;
 }
 
-// Per (lane, view) interpolation across the channels pf, pf + 1: fetch()
-// issues the loads of rows iq and iq + 1 from the byte offset of (view, iq,
-// pf), rows() returns their channel-interpolated values.
+// Per (lane, view) bilinear interpolation: fetch() issues the loads of rows
+// iq and iq + 1 at channels pf, pf + 1 from the byte offset of (view, iq, pf);
+// value() interpolates them across the channels and then between the rows
+// with the row fraction fq.
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 template <int PREC>
@@ -115,36 +116,79 @@ struct ChanInterp<FCT_PRECISION_FP32> {   // plain rows: two 8-byte loads
     return Raw{__builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0),
                __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, rowB, 0)};
   }
-  __device__ float2 rows(const Raw& t) const {
+  __device__ float value(const Raw& t, float fq) const {
     const uint32_t a0 = t.a.x, a1 = t.a.y, b0 = t.b.x, b1 = t.b.y;
     const float fa0 = __uint_as_float(a0), fa1 = __uint_as_float(a1);
     const float fb0 = __uint_as_float(b0), fb1 = __uint_as_float(b1);
-    return make_float2(fmaf(wp, fa1 - fa0, fa0), fmaf(wp, fb1 - fb0, fb0));
+    const float r0 = fmaf(wp, fa1 - fa0, fa0), r1 = fmaf(wp, fb1 - fb0, fb0);
+    return fmaf(fq, r1 - r0, r0);
   }
 };
+
+// a.lo * b.lo + a.hi * b.hi + c: two FP16 products (exact in FP32) and an
+// FP32 sum in one dot instruction.  (Through the builtin, not inline asm: a
+// dot result read by another VALU instruction needs wait states that only
+// the compiler's hazard recognizer inserts.)
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float dot2_f16(uint32_t a, uint32_t b, float c) {
+  return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, a), __builtin_bit_cast(half2_t, b), c,
+                                false);
+}
 
 template <>
 struct ChanInterp<FCT_PRECISION_FP16> {   // quad entries: one 8-byte load
   static constexpr uint32_t kEntry = 8;
   typedef u32x2 Raw;
-  float wp, omw;   // channel weights wp, 1 - wp
+  // Channel weights (1 - wq, wq) as packed FP16 and their negatives, with
+  // wq = fp16_channel_weight(wp): both weights are exact in FP16 and sum to 1
+  // (a position error below 2^-12 channel).
+  uint32_t w, nw;
   uint32_t base;
-  __device__ void init(uint32_t view_bytes, uint32_t pf, float w) {
-    wp = w;
-    omw = 1.0f - w;
+  __device__ void init(uint32_t view_bytes, uint32_t pf, float wp) {
+    const float wq = fp16_channel_weight(wp);
+    w = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(1.0f - wq, wq));
+    nw = w ^ 0x80008000u;
     base = view_bytes + pf * kEntry;
   }
   __device__ static Raw fetch(__amdgpu_buffer_rsrc_t r, uint32_t off, int) {
     return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
   }
-  __device__ float2 rows(const Raw& t) const {
-    // Through named scalars: hipcc (ROCm 7.2) folds __builtin_bit_cast of a
-    // vector element such as t[1] to element 0.  The products read the FP16
-    // halves directly (v_fma_mix_f32), FP32 weights and sums.
-    const uint32_t r0 = t.x, r1 = t.y;
-    const half2_t h0 = __builtin_bit_cast(half2_t, r0), h1 = __builtin_bit_cast(half2_t, r1);
-    return make_float2(fmaf((float)h0.y, wp, fmaf((float)h0.x, omw, 0.0f)),
-                       fmaf((float)h1.y, wp, fmaf((float)h1.x, omw, 0.0f)));
+  __device__ float value(const Raw& t, float fq) const {
+    // -r0 and r1 - r0 from the two rows' FP16 pairs, then r0 + fq (r1 - r0).
+    // (Through named scalars: hipcc (ROCm 7.2) folds __builtin_bit_cast of a
+    // vector element such as t[1] to element 0.)
+    const uint32_t q0 = t.x, q1 = t.y;
+    const float nr0 = dot2_f16(q0, nw, 0.0f);
+    const float d = dot2_f16(q1, w, nr0);
+    return fmaf(fq, d, -nr0);
+  }
+  // The same for four samples, as one block of eight v_dot2_f32_f16 in the
+  // VOP3P form (the builtin selects the two-address v_dot2c form and copies
+  // the accumulator).  Every dot result is read at least three instructions
+  // after it is written (the wait a dot result needs before another VALU
+  // instruction reads it): the second dot of a sample reads the first four
+  // instructions later, and s_nop 2 ends the block.
+  __device__ void values4(const Raw* t, const float* fq, float* val) const {
+    const uint32_t a0 = t[0].x, a1 = t[1].x, a2 = t[2].x, a3 = t[3].x;
+    const uint32_t b0 = t[0].y, b1 = t[1].y, b2 = t[2].y, b3 = t[3].y;
+    float n0, n1, n2, n3, d0, d1, d2, d3;
+    asm("v_dot2_f32_f16 %0, %8, %16, 0\n\t"
+        "v_dot2_f32_f16 %1, %9, %16, 0\n\t"
+        "v_dot2_f32_f16 %2, %10, %16, 0\n\t"
+        "v_dot2_f32_f16 %3, %11, %16, 0\n\t"
+        "v_dot2_f32_f16 %4, %12, %17, %0\n\t"
+        "v_dot2_f32_f16 %5, %13, %17, %1\n\t"
+        "v_dot2_f32_f16 %6, %14, %17, %2\n\t"
+        "v_dot2_f32_f16 %7, %15, %17, %3\n\t"
+        "s_nop 2"
+        : "=&v"(n0), "=&v"(n1), "=&v"(n2), "=&v"(n3), "=&v"(d0), "=&v"(d1), "=&v"(d2),
+          "=&v"(d3)
+        : "v"(a0), "v"(a1), "v"(a2), "v"(a3), "v"(b0), "v"(b1), "v"(b2), "v"(b3), "v"(nw),
+          "v"(w));
+    val[0] = fmaf(fq[0], d0, -n0);
+    val[1] = fmaf(fq[1], d1, -n1);
+    val[2] = fmaf(fq[2], d2, -n2);
+    val[3] = fmaf(fq[3], d3, -n3);
   }
 };
 
@@ -161,11 +205,12 @@ struct ChanInterp<FCT_PRECISION_BF16> {   // quad entries: one 8-byte load
   __device__ static Raw fetch(__amdgpu_buffer_rsrc_t r, uint32_t off, int) {
     return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
   }
-  __device__ float2 rows(const Raw& t) const {
-    const uint32_t r0 = t.x, r1 = t.y;
-    const float a0 = __uint_as_float(r0 << 16), a1 = __uint_as_float(r0 & 0xFFFF0000u);
-    const float b0 = __uint_as_float(r1 << 16), b1 = __uint_as_float(r1 & 0xFFFF0000u);
-    return make_float2(fmaf(wp, a1 - a0, a0), fmaf(wp, b1 - b0, b0));
+  __device__ float value(const Raw& t, float fq) const {
+    const uint32_t q0 = t.x, q1 = t.y;
+    const float a0 = __uint_as_float(q0 << 16), a1 = __uint_as_float(q0 & 0xFFFF0000u);
+    const float b0 = __uint_as_float(q1 << 16), b1 = __uint_as_float(q1 & 0xFFFF0000u);
+    const float r0 = fmaf(wp, a1 - a0, a0), r1 = fmaf(wp, b1 - b0, b0);
+    return fmaf(fq, r1 - r0, r0);
   }
 };
 
@@ -211,6 +256,22 @@ __device__ __forceinline__ BpTap<PREC> bp_fetch(float zsi, float invLh, float ro
   return s;
 }
 
+// Weight of a sample (0 when it is not selected): W(q) for the WFBP sum,
+// R / l for the matched projector's plain sum over the views.
+template <int PREC, int MODE>
+__device__ __forceinline__ float bp_weight(const BpTap<PREC>& s, float fR, const FctBpConsts& g) {
+  if (MODE == kBpMatched) return s.sel ? fR : 0.0f;
+  const float w = fct_w_of_row(s.qi, g);
+  return s.sel ? w : 0.0f;
+}
+
+// Accumulation of a sample of weight W (bp_weight) and interpolated value val.
+template <int MODE>
+__device__ __forceinline__ void bp_acc(float W, float val, float& num, float& den) {
+  num = fmaf(W, val, num);
+  if (MODE != kBpMatched) den += W;
+}
+
 template <int PREC, int MODE>
 __device__ __forceinline__ void bp_add(const BpTap<PREC>& s, const ChanInterp<PREC>& ci, float fR,
                                        const FctBpConsts& g, float& num, float& den, int& cnt) {
@@ -218,15 +279,30 @@ __device__ __forceinline__ void bp_add(const BpTap<PREC>& s, const ChanInterp<PR
     cnt += s.sel ? 1 : 0;
     return;
   }
-  const float2 lh = ci.rows(s.raw);
-  const float val = fmaf(s.fq, lh.y - lh.x, lh.x);
-  if (MODE == kBpMatched) {   // plain sum over the views, weight R / l
-    num = fmaf(s.sel ? fR : 0.0f, val, num);
+  bp_acc<MODE>(bp_weight<PREC, MODE>(s, fR, g), ci.value(s.raw, s.fq), num, den);
+}
+
+// Four samples of one view (the whole-slab path): FP16 through the dot block.
+template <int PREC, int MODE>
+__device__ __forceinline__ void bp_add4(const BpTap<PREC>* s, const ChanInterp<PREC>& ci,
+                                        float fR, const FctBpConsts& g, float* num, float* den,
+                                        int* cnt) {
+  if constexpr (PREC == FCT_PRECISION_FP16 && MODE != kBpCount) {
+    // The weights first: their arithmetic overlaps the tap loads.
+    typename ChanInterp<PREC>::Raw t[4];
+    float fq[4], W[4], val[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      t[i] = s[i].raw;
+      fq[i] = s[i].fq;
+      W[i] = bp_weight<PREC, MODE>(s[i], fR, g);
+    }
+    ci.values4(t, fq, val);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bp_acc<MODE>(W[i], val[i], num[i], den[i]);
   } else {
-    const float w = fct_w_of_row(s.qi, g);
-    const float W = s.sel ? w : 0.0f;
-    num = fmaf(W, val, num);
-    den += W;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bp_add<PREC, MODE>(s[i], ci, fR, g, num[i], den[i], cnt[i]);
   }
 }
 
@@ -372,9 +448,10 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
             for (int i = 0; i < 4; ++i)
               tap[i] = bp_fetch<PREC, MODE, CENTERED>(zs[h + i], invLh, row0, ci, rf, rowB,
                                                       chan_ok, g);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-              bp_add<PREC, MODE>(tap[i], ci, fR, g, num[h + i], den[h + i], cnt[h + i]);
+            // FP16: the four loads are issued before the weight arithmetic
+            // that waits for them (the scheduler would sink them otherwise).
+            if (PREC == FCT_PRECISION_FP16) __builtin_amdgcn_sched_barrier(0);
+            bp_add4<PREC, MODE>(tap, ci, fR, g, num + h, den + h, cnt + h);
           }
         } else {
 #pragma unroll

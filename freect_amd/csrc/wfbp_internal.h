@@ -33,6 +33,13 @@ struct RebinDev {
 //    (no overflow); it is kept in scale[0] of the scan's scale buffer.
 constexpr int kScaleElems = 2;   // [0] s, [1] max |raw| bits (rebin+filter scratch)
 
+// Channel weight of the FP16 tap interpolation: the fraction wp rounded to a
+// multiple of 2^-11, so that it and 1 - it are exact FP16 values summing to 1
+// (the backprojection multiplies the FP16 taps by them in v_dot2_f32_f16).
+__device__ __forceinline__ float fp16_channel_weight(float wp) {
+  return __builtin_rintf(wp * 2048.0f) * (1.0f / 2048.0f);
+}
+
 // Everything the rebin + filter kernel reads.
 struct RebinFilterArgs {
   const float* raw;          // [view][row][ncols]
