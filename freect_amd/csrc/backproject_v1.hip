@@ -214,6 +214,54 @@ struct ChanInterp<FCT_PRECISION_BF16> {   // quad entries: one 8-byte load
   }
 };
 
+// Per (voxel column, theta~): p, sqrt(R^2 - p^2), the in-plane offset f and
+// the cone-angle z term kb (fct_bp_column for the views of both parities,
+// which share p, sqrt and asin).
+struct BpColumn {
+  float p, sq, f, kb;
+};
+__device__ __forceinline__ BpColumn bp_column(float x, float y, float2 cs, const FctBpConsts& g) {
+  BpColumn c;
+  c.p = fmaf(x, cs.y, -(y * cs.x));
+  c.sq = sqrtf(fmaf(-c.p, c.p, g.R2));
+  c.f = fmaf(x, cs.x, y * cs.y);
+  c.kb = g.kappa * fct_asinf(c.p * g.invR);
+  return c;
+}
+
+// z-interval of |q| <= 1 relative to a view's table z for the voxels of a
+// wave's tile in the FOV: [min(-kbs - lt), max(-kbs + lt)] with kbs = kb,
+// lt = (sq - f) tan for parity 0 and kbs = -kb, lt = (sq + f) tan for parity
+// 1, returned as (min, max, min, max).  It depends on the tile and theta~
+// only, so bp_zbounds_kernel tabulates it once per plan.
+__device__ __forceinline__ float4 tile_zbounds(const BpColumn& c, bool active,
+                                               const FctBpConsts& g) {
+  const float BIG = 3.0e38f;
+  const float lt0 = (c.sq - c.f) * g.tanc, lt1 = (c.sq + c.f) * g.tanc;
+  float4 r;
+  r.x = wave_min(active ? -c.kb - lt0 : BIG);
+  r.y = wave_max(active ? -c.kb + lt0 : -BIG);
+  r.z = wave_min(active ? c.kb - lt1 : BIG);
+  r.w = wave_max(active ? c.kb + lt1 : -BIG);
+  return r;
+}
+
+// One wave per tile (all 64 lanes active, as the reductions need).
+__global__ __launch_bounds__(64) void bp_zbounds_kernel(BpArgs a, float4* out) {
+  const int tile = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+  const int ix = tx * 8 + (lane & 7), iy = ty * 8 + (lane >> 3);
+  const bool inb = ix < a.nx && iy < a.ny;
+  const float x = inb ? a.xs[ix] : 0.0f;
+  const float y = inb ? a.ys[iy] : 0.0f;
+  const bool active = inb && fct_in_fov(x, y, a.g);
+  for (int t = 0; t < a.H; ++t) {
+    const float4 r = tile_zbounds(bp_column(x, y, a.cs[t], a.g), active, a.g);
+    if (lane == 0) out[(size_t)tile * a.H + t] = r;
+  }
+}
+
 // One (voxel, view, slice) sample in two steps: bp_fetch computes the row
 // position and the exact selection and issues the tap loads; bp_add
 // interpolates and accumulates into (num, den) or cnt.  Branch-free: lanes
@@ -370,19 +418,18 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
     acc[i] = 0.0f;
     cnt[i] = 0;
   }
-  const float BIG = 3.0e38f;
-
   for (int t = 0; t < H; ++t) {
     const int kmin = vr.x <= t ? 0 : (vr.x - t + H - 1) / H;
     const int kmax = vr.y <= t ? 0 : (vr.y - t + H - 1) / H;
     if (kmin >= kmax) continue;
     const float* csp = reinterpret_cast<const float*>(a.cs + t);
     const float2 cs = make_float2(ld_uniform(csp), ld_uniform(csp + 1));
-    // fct_bp_column for (c, s) and (-c, -s): p, sqrt and asin are shared.
-    const float p = fmaf(x, cs.y, -(y * cs.x));
-    const float sq = sqrtf(fmaf(-p, p, g.R2));
-    const float f = fmaf(x, cs.x, y * cs.y);
-    const float kb = g.kappa * fct_asinf(p * g.invR);
+    const BpColumn col = bp_column(x, y, cs, g);
+    const float p = col.p, sq = col.sq, f = col.f, kb = col.kb;
+    // The tile's z-interval extremes for this theta~ (tabulated per plan).
+    const float* zbp = reinterpret_cast<const float*>(a.zbounds + (size_t)tile * H + t);
+    const float4 zb =
+        make_float4(ld_uniform(zbp), ld_uniform(zbp + 1), ld_uniform(zbp + 2), ld_uniform(zbp + 3));
     float num[TZ], den[TZ];
 #pragma unroll
     for (int i = 0; i < TZ; ++i) {
@@ -409,8 +456,8 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
       // (a row spans drow l / Dsd of z at distance l from the source).
       const float fR = MODE == kBpMatched ? g.tanc / (lt * g.invR) : 0.0f;
       // z-interval of |q| <= 1 relative to the view's table z, over the wave.
-      const float A = wave_min(active ? -kbs - lt : BIG);
-      const float B = wave_max(active ? -kbs + lt : -BIG);
+      const float A = par ? zb.z : zb.x;
+      const float B = par ? zb.w : zb.y;
       // Slice positions of the interval ends for a view at table z ztv:
       // ztv * inv_dzs + c, the lower end first (A <= B, so the order is
       // that of the slice step's sign).
@@ -629,6 +676,13 @@ hipError_t backproject_batch_waves_per_cu(int tile_z, int precision, int* waves)
   }
 #undef FCT_BATCH_K
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(waves, f, 64, 0);
+}
+
+hipError_t launch_bp_zbounds(const BpArgs& a, float4* zbounds, hipStream_t s) {
+  const int tiles = a.tiles_x * a.tiles_y;
+  if (tiles <= 0 || a.H <= 0) return hipSuccess;
+  hipLaunchKernelGGL(bp_zbounds_kernel, dim3(tiles), dim3(64), 0, s, a, zbounds);
+  return hipGetLastError();
 }
 
 hipError_t launch_backproject_v1(const BpArgs& a, int tile_z, int precision, int mode,

@@ -150,7 +150,7 @@ void free_plan_buffers(fct_wfbp_gpu_plan* p) {
                   p->d_slab_vrange, p->d_rebin,  p->d_Hn,        p->d_tw,       p->d_thk_off,
                   p->d_thk_idx,  p->d_thk_w,     p->d_filtered,  p->d_scale, p->d_native,
                   p->d_raw_stage, p->d_vol_stage, p->d_gbar, p->d_thkT_off, p->d_thkT_idx,
-                  p->d_thkT_w, p->d_wrange};
+                  p->d_thkT_w, p->d_wrange, p->d_zbounds};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
 }
@@ -215,6 +215,7 @@ BpArgs plan_bp_args(const fct_wfbp_gpu_plan* p) {
   a.zt = p->d_zt;
   a.slab_vrange = p->d_slab_vrange;
   a.wrange = p->d_wrange;
+  a.zbounds = p->d_zbounds;
   a.g = p->g;
   a.scale = p->scale;
   a.nx = p->recon.nx;
@@ -498,6 +499,14 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   PLAN_TRY(upload(&p->d_thkT_off, tToff, p->stream));
   PLAN_TRY(upload(&p->d_thkT_idx, tTidx, p->stream));
   PLAN_TRY(upload(&p->d_thkT_w, tTw, p->stream));
+  {
+    // Per-(tile, theta~) z-interval extremes of the backprojection (from the
+    // tables above, on the plan's stream).
+    const fct::BpArgs a = fct::plan_bp_args(p);
+    const size_t n = (size_t)a.tiles_x * a.tiles_y * a.H;
+    PLAN_TRY(hipMalloc(reinterpret_cast<void**>(&p->d_zbounds), n * sizeof(float4)));
+    PLAN_TRY(fct::launch_bp_zbounds(a, p->d_zbounds, p->stream));
+  }
 #undef PLAN_TRY
   *out = p;
   return FCT_OK;

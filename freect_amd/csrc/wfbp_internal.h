@@ -76,6 +76,11 @@ struct BpArgs {
   const float* zt;           // [nchunks][nviews] view z relative to the chunk reference
   const int2* slab_vrange;   // [nslabs] views [x, y) a slab can select
   const int2* wrange;        // [nchunks][nviews] native slices [x, y] inside the view window
+  // [tiles_y * tiles_x][H] per 8 x 8 voxel tile and theta~: the extremes over
+  // the tile's voxels in the FOV of the z-offsets from a view's table z at
+  // which |q| = 1, (min, max) for the view parity 0 then 1
+  // (launch_bp_zbounds).
+  const float4* zbounds;
   FctBpConsts g;
   float scale;               // 2 pi / ppr
   int32_t nx, ny, n_native, nviews, H, cpitch;
@@ -135,6 +140,9 @@ hipError_t launch_rebin_adjoint(const RebinAdjArgs& a, hipStream_t s);
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_backproject(const BpArgs& a, int tile_z, int variant, int precision, int mode,
                               hipStream_t s);
+// Fills a.zbounds (a BpArgs whose zbounds member is the output) for the
+// plan's tiles and theta~ (backproject_v1.hip).
+hipError_t launch_bp_zbounds(const BpArgs& a, float4* zbounds, hipStream_t s);
 hipError_t launch_backproject_v1(const BpArgs& a, int tile_z, int precision, int mode,
                                  hipStream_t s);
 hipError_t launch_backproject_batch(const BpBatchArgs& b, int tile_z, int precision,
@@ -188,6 +196,7 @@ struct fct_wfbp_gpu_plan {
   float* d_zt = nullptr;
   int2* d_slab_vrange = nullptr;
   int2* d_wrange = nullptr;
+  float4* d_zbounds = nullptr;
   fct::RebinDev* d_rebin = nullptr;
   float* d_Hn = nullptr;
   float2* d_tw = nullptr;
