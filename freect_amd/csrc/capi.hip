@@ -547,6 +547,10 @@ FCT_API int fct_wfbp_gpu_plan_info(const fct_wfbp_gpu_plan* p, fct_plan_info* in
 
 FCT_API int fct_wfbp_gpu_set_stream(fct_wfbp_gpu_plan* p, void* stream) {
   if (!p) return FCT_ERR_INVALID_ARGUMENT;
+  // Work already queued on the old stream (the set-up tables of plan
+  // creation, earlier stages) completes before the plan moves.
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  HIP_TRY(p->ctx, hipStreamSynchronize(p->stream));
   p->stream = static_cast<hipStream_t>(stream);
   return FCT_OK;
 }

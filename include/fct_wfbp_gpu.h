@@ -160,6 +160,8 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
                                      fct_wfbp_gpu_plan** plan);
 FCT_API int fct_wfbp_gpu_plan_destroy(fct_wfbp_gpu_plan* plan);
 FCT_API int fct_wfbp_gpu_plan_info(const fct_wfbp_gpu_plan* plan, fct_plan_info* info);
+/* Moves the plan to another stream after its queued work on the current one
+ * (plan set-up included) has completed. */
 FCT_API int fct_wfbp_gpu_set_stream(fct_wfbp_gpu_plan* plan, void* stream);
 
 /* Rebin + ramp filter (ReconstructionMachine.cu:141-306, cuda_kernels.cuh:11-77).

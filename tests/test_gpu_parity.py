@@ -339,6 +339,26 @@ def test_small_workload_end_to_end(ctx, oracle):
     assert p.count_voxel_views() == n_sel
 
 
+def test_plan_moved_to_another_stream(ctx):
+    # Plan creation queues the backprojection's z-interval table on the
+    # plan's stream; set_stream waits for it, so a plan moved at once to
+    # another stream reconstructs bitwise what a plan kept on one stream does.
+    w = S.tiny(nx=48, n_slices=12, n_channels=192, n_rows=16, ppr=192, rotations=3)
+    raw_t = _raw_gpu(ctx, w)
+    torch.cuda.synchronize()
+    ref_plan = _plan(ctx, w, precision="fp16")
+    ref = ref_plan.alloc_volume()
+    ref_plan.reconstruct(raw_t, ref)
+    ref_plan.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    p = _plan(ctx, w, precision="fp16", stream=s1)
+    p.set_stream(s2)
+    vol = p.alloc_volume()
+    p.reconstruct(raw_t, vol)
+    p.synchronize()
+    assert torch.equal(vol, ref)
+
+
 @pytest.mark.parametrize("precision,rmse_bound", [("fp16", 0.5), ("bf16", 8.0)])
 def test_mixed_precision_error(ctx, oracle, precision, rmse_bound):
     w = S.tiny(nx=64, n_slices=16, n_channels=192, n_rows=16, ppr=192, rotations=3)
