@@ -540,17 +540,18 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   }
 }
 
-// Occupancy floors for the 16-bit kernels: at least seven waves per SIMD at
-// TZ = 8 and six at TZ = 16 (the registers of five otherwise; the spills this
-// costs sit in the per-theta~ setup, outside the sample loop).
+// Occupancy floor for the 16-bit kernels at TZ <= 8: at least seven waves
+// per SIMD (the few spills this costs sit in the per-theta~ setup, outside
+// the sample loop).  At TZ = 16 a floor of six waves spills inside the view
+// loop, so the allocator's own choice (five) stands.
 template <int TZ, int PREC>
 constexpr int bp_min_waves() {
-  return PREC == FCT_PRECISION_FP32 ? 1 : TZ <= 8 ? 7 : TZ == 16 ? 6 : 1;
+  return PREC == FCT_PRECISION_FP32 ? 1 : TZ <= 8 ? 7 : 1;
 }
 
 template <int TZ, int PREC, int MODE, bool CENTERED>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(TZ == 8 || TZ == 16 ? bp_min_waves<TZ, PREC>() : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(TZ == 8 ? bp_min_waves<TZ, PREC>() : 1, 8)))
 void bp_v1_kernel(BpArgs a) {
   bp_v1_tile<TZ, PREC, MODE, CENTERED>(a, xcd_block(blockIdx.x, gridDim.x));
 }
