@@ -220,10 +220,38 @@ struct ChanInterp<FCT_PRECISION_BF16> {   // quad entries: one 8-byte load
 struct BpColumn {
   float p, sq, f, kb;
 };
+
+// sqrt(x) correctly rounded, as sqrtf, for x >= 2^-96 (R^2 - p^2 here is
+// of the order of R^2): the hardware square root (within 1 ulp) moved to
+// its neighbour below or above when the residual x - s' s of that neighbour
+// says the true root lies beyond it.  sqrtf's expansion adds a rescaling of
+// tiny inputs and the zero / infinity cases, which cannot occur here.
+__device__ __forceinline__ float sqrt_rn_large(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sdn = __int_as_float(__float_as_int(s) - 1);
+  const float sup = __int_as_float(__float_as_int(s) + 1);
+  float r = fmaf(-sdn, s, x) <= 0.0f ? sdn : s;
+  r = fmaf(-sup, s, x) > 0.0f ? sup : r;
+  return r;
+}
+
+// n / d correctly rounded, as the IEEE division, for operands far from the
+// range limits (here n = hr < 2^15 and d = lt, a cone length of millimetres
+// times tan(cone / 2)): the hardware reciprocal refined once, the quotient
+// corrected twice by its exact residual -- the division's own sequence
+// without its operand scaling and special-case fix-up, identities for such
+// operands.
+__device__ __forceinline__ float div_rn_normal(float n, float d) {
+  float r = __builtin_amdgcn_rcpf(d);
+  r = fmaf(fmaf(-d, r, 1.0f), r, r);
+  float q = n * r;
+  q = fmaf(fmaf(-d, q, n), r, q);
+  return fmaf(fmaf(-d, q, n), r, q);
+}
 __device__ __forceinline__ BpColumn bp_column(float x, float y, float2 cs, const FctBpConsts& g) {
   BpColumn c;
   c.p = fmaf(x, cs.y, -(y * cs.x));
-  c.sq = sqrtf(fmaf(-c.p, c.p, g.R2));
+  c.sq = sqrt_rn_large(fmaf(-c.p, c.p, g.R2));
   c.f = fmaf(x, cs.x, y * cs.y);
   c.kb = g.kappa * fct_asinf(c.p * g.invR);
   return c;
@@ -446,7 +474,7 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
       if (k >= kmax) continue;
       const float pidx = par ? fmaf(-p, g.inv_ds, g.cc) : fmaf(p, g.inv_ds, g.cc);
       const float lt = (par ? sq + f : sq - f) * g.tanc;
-      const float invLh = g.hr / lt;
+      const float invLh = div_rn_normal(g.hr, lt);
       const float kbs = par ? -kb : kb;
       const float pfl = floorf(pidx);
       const bool chan_ok = active && pfl >= 0.0f && pfl <= (float)(g.ncols - 2);
