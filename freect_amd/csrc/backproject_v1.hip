@@ -461,11 +461,17 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
     float num[TZ], den[TZ];
 #pragma unroll
     for (int i = 0; i < TZ; ++i) {
-      num[i] = 0.0f;
       // 1e-30 is below half an ulp of any nonzero weight sum (W >= ~1e-13
       // when a sample is selected), so den ends as the exact sum when a
       // view contributes and 1e-30 (num = 0, no change to acc) otherwise.
-      den[i] = 1e-30f;
+      // (Through an empty asm: as literals the compiler materialised both
+      // start values again at the view loop's entry, 32 instead of 14 moves
+      // per theta~.)
+      float zero = 0.0f, tiny = 1e-30f;
+      asm("" : "+v"(zero));
+      asm("" : "+v"(tiny));
+      num[i] = zero;
+      den[i] = tiny;
     }
 
 #pragma unroll
