@@ -534,6 +534,23 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
             if (PREC == FCT_PRECISION_FP16) __builtin_amdgcn_sched_barrier(0);
             bp_add4<PREC, MODE>(tap, ci, fR, g, num + h, den + h, cnt + h);
           }
+        } else if constexpr (PREC == FCT_PRECISION_FP16) {
+          // Part of the slab, FP16: groups of four slices that meet
+          // [ilo, ihi] through the dot block, the slices outside the range
+          // masked like unselected samples (cheaper than per-slice samples
+          // with the two-address dot form; for FP32 / BF16 the masked slices
+          // cost more than they save).
+#pragma unroll
+          for (int h = 0; h < TZ; h += 4) {
+            if (ilo > h + 3 || ihi < h) continue;
+            BpTap<PREC> tap[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              tap[i] = bp_fetch<PREC, MODE, CENTERED>(zs[h + i], invLh, row0, ci, rf, rowB,
+                                                      chan_ok && h + i >= ilo && h + i <= ihi, g);
+            __builtin_amdgcn_sched_barrier(0);
+            bp_add4<PREC, MODE>(tap, ci, fR, g, num + h, den + h, cnt + h);
+          }
         } else {
 #pragma unroll
           for (int i = 0; i < TZ; ++i)
