@@ -12,9 +12,10 @@
 //     per-sample selection still runs, so selection is unchanged;
 //   * the views of one theta~ and parity move monotonically in z, so the
 //     view loop ends at the first view whose slice range lies past the slab;
-//   * when a view covers the whole slab (the common case) all slices are
-//     processed without a per-slice branch, so the taps of every slice are in
-//     flight at once;
+//   * when a view covers the whole slab (the common case) the slices are
+//     processed four at a time without a per-slice branch, the taps of the
+//     four in flight together (with FP16 taps also for views covering part of
+//     the slab, the slices outside the view's range masked);
 //   * the detector row position is one fma per slice, and when the central
 //     row is (nrows-1)/2 the cone test |q| <= 1 is implied exactly by the row
 //     test, one unsigned compare;
