@@ -5,10 +5,11 @@
 // instructions per selected (voxel, view) sample -- the kernel is bound by
 // instruction issue (VALU and vector-memory), not by HBM:
 //   * per (theta~, view parity) each lane's z-interval of |q| <= 1 is
-//     [zt - kb - lt, zt - kb + lt]; a DPP wave reduction of its ends gives,
-//     per view, the slices any lane of the wave can select, intersected with
-//     the view window's exact slice range (a host table built with the same
-//     FP32 test); other slices are skipped wave-uniformly and the exact
+//     [zt - kb - lt, zt - kb + lt]; the extremes of its ends over the tile
+//     (DPP wave reductions, tabulated once per plan by bp_zbounds_kernel)
+//     give, per view, the slices any lane of the wave can select, intersected
+//     with the view window's exact slice range (a host table built with the
+//     same FP32 test); other slices are skipped wave-uniformly and the exact
 //     per-sample selection still runs, so selection is unchanged;
 //   * the views of one theta~ and parity move monotonically in z, so the
 //     view loop ends at the first view whose slice range lies past the slab;
