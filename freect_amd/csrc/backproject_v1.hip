@@ -20,13 +20,14 @@
 //   * the detector row position is one fma per slice, and when the central
 //     row is (nrows-1)/2 the cone test |q| <= 1 is implied exactly by the row
 //     test, one unsigned compare;
-//   * taps: FP32 rows are two 8-byte buffer loads (the second row through the
-//     scalar offset); FP16 / BF16 projections are stored as 2 x 2 quads, one
-//     8-byte load per sample, and scaled once per scan at the end; FP16 quads
-//     are interpolated across channels by two v_dot2_f32_f16 with the channel
+//   * taps: FP32 rows are two 8-byte buffer loads, each at its own vector
+//     offset; FP16 / BF16 projections are stored as 2 x 2 quads, one 8-byte
+//     load per sample, and scaled once per scan at the end; FP16 quads are
+//     interpolated across channels by two v_dot2_f32_f16 with the channel
 //     weights packed as FP16 (exact products, FP32 sums); lanes that select
-//     nothing read whatever the range-checked buffer returns and get weight 0
-//     instead of a branch;
+//     nothing read whatever the range-checked buffer returns (filtered
+//     values of the slab, or 0 outside it) and get weight 0 instead of a
+//     branch;
 //   * the per-theta~ mean uses v_rcp.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -99,6 +100,12 @@ __device__ __forceinline__ T ld_uniform(const T* p) {
 // with the row fraction fq.
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
+// The filtered projections of a slab as a buffer resource based at the
+// slab's first view.
+struct BpRsrc {
+  __amdgpu_buffer_rsrc_t rows;
+};
+
 template <int PREC>
 struct ChanInterp;
 
@@ -114,9 +121,12 @@ struct ChanInterp<FCT_PRECISION_FP32> {   // plain rows: two 8-byte loads
     wp = w;
     base = view_bytes + pf * kEntry;
   }
-  __device__ static Raw fetch(__amdgpu_buffer_rsrc_t r, uint32_t off, int rowB) {
-    return Raw{__builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0),
-               __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, rowB, 0)};
+  // Row iq + 1 at the vector offset off + rowB: both loads are range-checked
+  // (the hardware's check covers the vector offset, not the scalar one), so
+  // lanes that select nothing never read past the slab's projections.
+  __device__ static Raw fetch(const BpRsrc& r, uint32_t off, uint32_t rowB) {
+    return Raw{__builtin_amdgcn_raw_buffer_load_b64(r.rows, (int)off, 0, 0),
+               __builtin_amdgcn_raw_buffer_load_b64(r.rows, (int)(off + rowB), 0, 0)};
   }
   __device__ float value(const Raw& t, float fq) const {
     const uint32_t a0 = t.a.x, a1 = t.a.y, b0 = t.b.x, b1 = t.b.y;
@@ -152,8 +162,8 @@ struct ChanInterp<FCT_PRECISION_FP16> {   // quad entries: one 8-byte load
     nw = w ^ 0x80008000u;
     base = view_bytes + pf * kEntry;
   }
-  __device__ static Raw fetch(__amdgpu_buffer_rsrc_t r, uint32_t off, int) {
-    return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+  __device__ static Raw fetch(const BpRsrc& r, uint32_t off, uint32_t) {
+    return __builtin_amdgcn_raw_buffer_load_b64(r.rows, (int)off, 0, 0);
   }
   __device__ float value(const Raw& t, float fq) const {
     // -r0 and r1 - r0 from the two rows' FP16 pairs, then r0 + fq (r1 - r0).
@@ -204,8 +214,8 @@ struct ChanInterp<FCT_PRECISION_BF16> {   // quad entries: one 8-byte load
     wp = w;
     base = view_bytes + pf * kEntry;
   }
-  __device__ static Raw fetch(__amdgpu_buffer_rsrc_t r, uint32_t off, int) {
-    return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+  __device__ static Raw fetch(const BpRsrc& r, uint32_t off, uint32_t) {
+    return __builtin_amdgcn_raw_buffer_load_b64(r.rows, (int)off, 0, 0);
   }
   __device__ float value(const Raw& t, float fq) const {
     const uint32_t q0 = t.x, q1 = t.y;
@@ -315,7 +325,7 @@ __device__ __forceinline__ int floor_int(float x) {
 template <int PREC, int MODE, bool CENTERED>
 __device__ __forceinline__ BpTap<PREC> bp_fetch(float zsi, float invLh, float row0,
                                                 const ChanInterp<PREC>& ci,
-                                                __amdgpu_buffer_rsrc_t rf, uint32_t rowB,
+                                                const BpRsrc& rf, uint32_t rowB,
                                                 bool chan_ok, const FctBpConsts& g) {
   BpTap<PREC> s;
   s.qi = fmaf(zsi, invLh, row0);   // fct_bp_row
@@ -330,7 +340,7 @@ __device__ __forceinline__ BpTap<PREC> bp_fetch(float zsi, float invLh, float ro
     s.sel = chan_ok && !(fabsf(s.qi - g.cr) > g.hr) && iq >= 0 && iq <= g.nrows - 2;
   // 24-bit multiply: exact for selected rows (< 2^24).
   if (MODE != kBpCount)
-    s.raw = ChanInterp<PREC>::fetch(rf, ci.base + __umul24((uint32_t)iq, rowB), (int)rowB);
+    s.raw = ChanInterp<PREC>::fetch(rf, ci.base + __umul24((uint32_t)iq, rowB), rowB);
   return s;
 }
 
@@ -438,8 +448,8 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   const char* fb = static_cast<const char*>(a.filt) + (size_t)vr.x * vstride * kEntry;
   const uint32_t fbytes =
       (uint32_t)min((size_t)(vr.y - vr.x) * vstride * kEntry, (size_t)0xFFFFFFF0u);
-  const __amdgpu_buffer_rsrc_t rf =
-      __builtin_amdgcn_make_buffer_rsrc((void*)fb, 0, fbytes, kRsrcWord3);
+  BpRsrc rf;
+  rf.rows = __builtin_amdgcn_make_buffer_rsrc((void*)fb, 0, fbytes, kRsrcWord3);
 
   float acc[TZ];
   int cnt[TZ];

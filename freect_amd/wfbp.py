@@ -154,6 +154,42 @@ def _ptr(t) -> Optional[int]:
     return t.data_ptr()
 
 
+_DTYPE_NAMES = {"f32": "float32", "i32": "int32"}
+
+
+def _dev_arg(t, what: str, device: int, *, numel: Optional[int] = None,
+             nbytes: Optional[int] = None, dtype: Optional[str] = None,
+             optional: bool = False) -> Optional[int]:
+    """Checks a device-buffer argument before its address crosses the C ABI
+    (whose kernels trust the sizes of the plan): a contiguous CUDA tensor on
+    the plan's device, of the given element type ('f32' / 'i32') and element
+    count, or of exactly `nbytes` bytes for the opaque filtered buffer.  An
+    int is taken as a raw device address and passed through unchecked (the
+    C ABI's own contract); None is allowed only where the C ABI allows NULL."""
+    if t is None:
+        if optional:
+            return None
+        raise FctError(f"{what}: a device buffer is required")
+    if isinstance(t, int):
+        return t
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise FctError(f"{what}: expected a torch tensor or a device address, got {type(t)}")
+    if dtype is not None:
+        want = {"f32": torch.float32, "i32": torch.int32}[dtype]
+        if t.dtype != want:
+            raise FctError(f"{what}: dtype must be {_DTYPE_NAMES[dtype]} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise FctError(f"{what}: must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise FctError(f"{what}: {t.numel()} elements, the plan needs {numel}")
+    if nbytes is not None and t.numel() * t.element_size() != nbytes:
+        raise FctError(f"{what}: {t.numel() * t.element_size()} bytes, the plan needs {nbytes}")
+    if not t.is_cuda or t.device.index != device:
+        raise FctError(f"{what}: must live on cuda:{device} (got {t.device})")
+    return t.data_ptr()
+
+
 def _stream_handle(stream=None) -> int:
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -308,22 +344,47 @@ class Plan:
         self.ctx.check(load_library().fct_wfbp_gpu_set_stream(self.h, _stream_handle(stream)),
                        "set_stream")
 
+    # Checked arguments: every buffer against the plan's sizes (_dev_arg).
+    def _raw(self, t, what="raw"):
+        return _dev_arg(t, what, self.ctx.device, numel=self.info.raw_elems, dtype="f32")
+
+    def _filt(self, t, what="filtered"):
+        return _dev_arg(t, what, self.ctx.device, nbytes=self.info.filtered_bytes)
+
+    def _scale(self, t, what="scale"):
+        return _dev_arg(t, what, self.ctx.device, numel=self.info.scale_elems or None,
+                        dtype="f32", optional=True)
+
+    def _native(self, t, what="native"):
+        return _dev_arg(t, what, self.ctx.device, numel=self.info.native_elems, dtype="f32")
+
+    def _volume(self, t, what="volume"):
+        return _dev_arg(t, what, self.ctx.device, numel=self.info.volume_elems, dtype="f32")
+
+    def _lines(self, t, what="lines"):
+        i = self.info
+        return _dev_arg(t, what, self.ctx.device,
+                        numel=i.n_parallel_views * i.n_filtered_rows * i.channel_pitch,
+                        dtype="f32")
+
     def rebin_filter(self, raw, filt, scale=None):
-        self.ctx.check(load_library().fct_wfbp_gpu_rebin_filter(self.h, _ptr(raw), _ptr(filt),
-                                                                _ptr(scale)), "rebin_filter")
+        self.ctx.check(load_library().fct_wfbp_gpu_rebin_filter(
+            self.h, self._raw(raw), self._filt(filt), self._scale(scale)), "rebin_filter")
 
     def backproject(self, filt, scale, native):
-        self.ctx.check(load_library().fct_wfbp_gpu_backproject(self.h, _ptr(filt), _ptr(scale),
-                                                              _ptr(native)), "backproject")
+        self.ctx.check(load_library().fct_wfbp_gpu_backproject(
+            self.h, self._filt(filt), self._scale(scale), self._native(native)), "backproject")
 
     def thicken(self, native, volume):
-        self.ctx.check(load_library().fct_wfbp_gpu_thicken(self.h, _ptr(native), _ptr(volume)),
-                       "thicken")
+        self.ctx.check(load_library().fct_wfbp_gpu_thicken(
+            self.h, self._native(native), self._volume(volume)), "thicken")
 
     def reconstruct(self, raw, volume, host_buffers: bool = False):
-        self.ctx.check(load_library().fct_wfbp_gpu_reconstruct(self.h, _ptr(raw), _ptr(volume),
-                                                              1 if host_buffers else 0),
-                       "reconstruct")
+        if host_buffers:
+            raise FctError("reconstruct(host_buffers=True) takes numpy arrays: use "
+                           "reconstruct_host()")
+        self.ctx.check(load_library().fct_wfbp_gpu_reconstruct(
+            self.h, self._raw(raw), self._volume(volume), 0), "reconstruct")
 
     def reconstruct_host(self, raw: np.ndarray) -> np.ndarray:
         raw = np.ascontiguousarray(raw, np.float32)
@@ -347,38 +408,41 @@ class Plan:
 
     def backproject_adjoint(self, native_bar, lines_bar):
         self.ctx.check(load_library().fct_wfbp_gpu_backproject_adjoint(
-            self.h, _ptr(native_bar), _ptr(lines_bar)), "backproject_adjoint")
+            self.h, self._native(native_bar, "native_bar"), self._lines(lines_bar, "lines_bar")),
+            "backproject_adjoint")
 
     def filter_lines(self, lines_in, lines_out):
-        self.ctx.check(load_library().fct_wfbp_gpu_filter(self.h, _ptr(lines_in), _ptr(lines_out)),
-                       "filter")
+        self.ctx.check(load_library().fct_wfbp_gpu_filter(
+            self.h, self._lines(lines_in, "lines_in"), self._lines(lines_out, "lines_out")),
+            "filter")
 
     def rebin_lines(self, raw, lines_out):
         """Rebinning without the filter: FP32 [view][row][channel_pitch]."""
-        self.ctx.check(load_library().fct_wfbp_gpu_rebin(self.h, _ptr(raw), _ptr(lines_out)),
-                       "rebin")
+        self.ctx.check(load_library().fct_wfbp_gpu_rebin(
+            self.h, self._raw(raw), self._lines(lines_out, "lines_out")), "rebin")
 
     def matched_project(self, native, lines):
-        self.ctx.check(load_library().fct_wfbp_gpu_matched_project(self.h, _ptr(native),
-                                                                  _ptr(lines)), "matched_project")
+        self.ctx.check(load_library().fct_wfbp_gpu_matched_project(
+            self.h, self._native(native), self._lines(lines)), "matched_project")
 
     def matched_backproject(self, lines, native):
-        self.ctx.check(load_library().fct_wfbp_gpu_matched_backproject(self.h, _ptr(lines),
-                                                                      _ptr(native)),
-                       "matched_backproject")
+        self.ctx.check(load_library().fct_wfbp_gpu_matched_backproject(
+            self.h, self._lines(lines), self._native(native)), "matched_backproject")
 
     def rebin_adjoint(self, lines_bar, raw_bar):
-        self.ctx.check(load_library().fct_wfbp_gpu_rebin_adjoint(self.h, _ptr(lines_bar),
-                                                                _ptr(raw_bar)), "rebin_adjoint")
+        self.ctx.check(load_library().fct_wfbp_gpu_rebin_adjoint(
+            self.h, self._lines(lines_bar, "lines_bar"), self._raw(raw_bar, "raw_bar")),
+            "rebin_adjoint")
 
     def thicken_adjoint(self, volume_bar, native_bar):
-        self.ctx.check(load_library().fct_wfbp_gpu_thicken_adjoint(self.h, _ptr(volume_bar),
-                                                                  _ptr(native_bar)),
-                       "thicken_adjoint")
+        self.ctx.check(load_library().fct_wfbp_gpu_thicken_adjoint(
+            self.h, self._volume(volume_bar, "volume_bar"), self._native(native_bar, "native_bar")),
+            "thicken_adjoint")
 
     def adjoint(self, native_bar, raw_bar):
-        self.ctx.check(load_library().fct_wfbp_gpu_adjoint(self.h, _ptr(native_bar),
-                                                          _ptr(raw_bar)), "adjoint")
+        self.ctx.check(load_library().fct_wfbp_gpu_adjoint(
+            self.h, self._native(native_bar, "native_bar"), self._raw(raw_bar, "raw_bar")),
+            "adjoint")
 
     def forward_native(self, raw, native, filt=None):
         """raw -> native volume (rebin + filter + backproject), FP32 plan."""
@@ -390,7 +454,9 @@ class Plan:
 
     def count_voxel_views(self, per_voxel=None) -> int:
         tot = C.c_int64(0)
-        self.ctx.check(load_library().fct_wfbp_gpu_count_voxel_views(self.h, _ptr(per_voxel),
+        pv = _dev_arg(per_voxel, "per_voxel", self.ctx.device, numel=self.info.native_elems,
+                      dtype="i32", optional=True)
+        self.ctx.check(load_library().fct_wfbp_gpu_count_voxel_views(self.h, pv,
                                                                     C.byref(tot)),
                        "count_voxel_views")
         return tot.value
@@ -456,8 +522,11 @@ class Batch:
         return {k: getattr(i, k) for k, _ in BatchInfoC._fields_}
 
     def submit(self, job: int, raw, volume):
-        self.ctx.check(load_library().fct_wfbp_gpu_batch_submit(self.h, job, _ptr(raw),
-                                                                _ptr(volume)), "batch_submit")
+        if not 0 <= job < len(self.plans):
+            raise FctError(f"batch_submit: job {job} outside [0, {len(self.plans)})")
+        p = self.plans[job]
+        self.ctx.check(load_library().fct_wfbp_gpu_batch_submit(
+            self.h, job, p._raw(raw), p._volume(volume)), "batch_submit")
 
     def run(self, mode: int = 0):
         self.ctx.check(load_library().fct_wfbp_gpu_batch_run(self.h, mode), "batch_run")

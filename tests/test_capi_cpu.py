@@ -55,3 +55,31 @@ def test_create_fails_cleanly_without_gpu():
     h = C.c_void_p()
     st = wfbp.load_library().fct_wfbp_gpu_create(0, C.byref(h))
     assert st in (1, 4) and not h.value
+
+
+def test_buffer_arguments_are_checked_before_the_abi():
+    # Every tensor handed to a plan stage is checked against the plan's sizes
+    # before its address crosses the C ABI: element type, contiguity, element
+    # count (or byte count for the opaque filtered buffer) and device.
+    import torch
+    from freect_amd import wfbp
+    chk = wfbp._dev_arg
+    good = torch.zeros(12, dtype=torch.float32)
+    with pytest.raises(wfbp.FctError, match="dtype must be float32"):
+        chk(good.double(), "raw", 0, numel=12, dtype="f32")
+    with pytest.raises(wfbp.FctError, match="dtype must be int32"):
+        chk(good, "per_voxel", 0, numel=12, dtype="i32")
+    with pytest.raises(wfbp.FctError, match="contiguous"):
+        chk(torch.zeros(4, 6)[:, :3], "raw", 0, numel=12, dtype="f32")
+    with pytest.raises(wfbp.FctError, match="11 elements, the plan needs 12"):
+        chk(good[:11], "raw", 0, numel=12, dtype="f32")
+    with pytest.raises(wfbp.FctError, match="48 bytes, the plan needs 40"):
+        chk(good, "filtered", 0, nbytes=40)
+    with pytest.raises(wfbp.FctError, match="must live on cuda:0"):
+        chk(good, "raw", 0, numel=12, dtype="f32")
+    with pytest.raises(wfbp.FctError, match="a device buffer is required"):
+        chk(None, "native", 0, numel=12, dtype="f32")
+    with pytest.raises(wfbp.FctError, match="expected a torch tensor"):
+        chk(good.numpy(), "raw", 0, numel=12, dtype="f32")
+    assert chk(None, "scale", 0, optional=True) is None
+    assert chk(1234, "raw", 0, numel=12, dtype="f32") == 1234   # raw address: the C contract
