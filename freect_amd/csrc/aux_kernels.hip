@@ -69,7 +69,8 @@ __device__ double poisson(double lam, RayRng& rng) {
 
 // One ray per thread: source S = (R cos a, R sin a, z_v), fan angle
 // beta = (c - cc) dgamma, detector cell at in-plane distance Dsd and height
-// (r - cr) drow.  Analytic chords of an infinite water cylinder along z and of
+// (r - cr) drow.  A clockwise scan traces the counter-clockwise ray at
+// angle -a_v and mirrors it (y -> -y).  Analytic chords of an infinite water cylinder along z and of
 // rotated ellipsoids, in FP64.
 __global__ __launch_bounds__(256) void phantom_kernel(PhantomArgs a) {
   const int c = blockIdx.x * 256 + threadIdx.x;
@@ -104,6 +105,10 @@ __global__ __launch_bounds__(256) void phantom_kernel(PhantomArgs a) {
     ux = ex / len;
     uy = ey / len;
     uz = ez / len;
+  }
+  if (a.mirror) {   // clockwise: the mirror image of the ray traced above
+    sy = -sy;
+    uy = -uy;
   }
   double acc = 0.0;
   {

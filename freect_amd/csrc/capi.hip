@@ -117,6 +117,9 @@ int validate(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* s, const fct_recon_
     return set_err(ctx, "table feed must be nonzero (helical scans only)"), FCT_ERR_UNSUPPORTED;
   if (s->ffs_mode < FCT_FFS_NONE || s->ffs_mode > FCT_FFS_PHI_Z)
     return set_err(ctx, "unknown ffs_mode %d", s->ffs_mode), FCT_ERR_INVALID_ARGUMENT;
+  if (s->rotation < -1 || s->rotation > 1)
+    return set_err(ctx, "rotation must be +1 (or 0) or -1 (got %d)", s->rotation),
+           FCT_ERR_INVALID_ARGUMENT;
   if (s->ffs_mode != FCT_FFS_NONE) {
     const int S = fct_ffs::n_states(s->ffs_mode);
     if (s->views_per_rotation % (2 * S))
@@ -364,15 +367,23 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   p->nchunks = (p->n_native + chunk - 1) / chunk;
   p->nslabs = (p->n_native + p->tile_z - 1) / p->tile_z;
 
+  // A clockwise scan is the mirror image y -> -y of the counter-clockwise
+  // scan with tube angles -a_v: the kernels reconstruct that one at the
+  // mirrored voxel centres (-y), which is the clockwise volume in place.
+  const bool cw = s.rotation < 0;
   std::vector<float> xs(r.nx), ys(r.ny), zs(p->n_native), zt((size_t)p->nchunks * p->nviews);
   for (int i = 0; i < r.nx; ++i) xs[i] = fct_tables::voxel_coord(i, r.nx, r.recon_fov, r.x_origin);
-  for (int i = 0; i < r.ny; ++i) ys[i] = fct_tables::voxel_coord(i, r.ny, r.recon_fov, r.y_origin);
+  for (int i = 0; i < r.ny; ++i) {
+    ys[i] = fct_tables::voxel_coord(i, r.ny, r.recon_fov, r.y_origin);
+    if (cw) ys[i] = -ys[i];
+  }
   for (int i = 0; i < p->n_native; ++i) zs[i] = (float)(zn[i] - zn[(i / chunk) * chunk]);
   for (int c = 0; c < p->nchunks; ++c)
     for (int v = 0; v < p->nviews; ++v)
       zt[(size_t)c * p->nviews + v] = fct_tables::view_z_rel(s.z0, p->dz, v, zn[c * chunk]);
   std::vector<float2> cs(p->H);
-  for (int t = 0; t < p->H; ++t) fct_tables::view_cos_sin(s.angle0, p->dtheta, t, &cs[t].x, &cs[t].y);
+  const double angle0 = cw ? -s.angle0 : s.angle0;
+  for (int t = 0; t < p->H; ++t) fct_tables::view_cos_sin(angle0, p->dtheta, t, &cs[t].x, &cs[t].y);
 
   std::vector<int2> vr;
   if (!slab_view_ranges(p, zn, p->tile_z, &vr))
@@ -897,7 +908,10 @@ FCT_API int fct_wfbp_gpu_phantom_project(fct_wfbp_gpu_ctx* ctx, const fct_scan_g
   a.drow = scan->row_spacing;
   a.cc = scan->central_channel;
   a.cr = scan->central_row;
-  a.angle0 = scan->angle0;
+  // Clockwise: the rays of the counter-clockwise scan at angles -a_v,
+  // mirrored y -> -y by the kernel.
+  a.mirror = scan->rotation < 0 ? 1 : 0;
+  a.angle0 = a.mirror ? -scan->angle0 : scan->angle0;
   a.dtheta = 2.0 * kPi / (double)scan->views_per_rotation;
   a.z0 = scan->z0;
   a.dz = scan->feed / (double)scan->views_per_rotation;

@@ -151,7 +151,7 @@ int parse_config(const std::string& config_file, ReconConfig& rp, bool verbose, 
             get("recon_kernel", rp.recon_kernel) && get("x_origin", rp.x_origin) &&
             get("y_origin", rp.y_origin) && get("tube_angle_offset", rp.tube_angle_offset) &&
             get("adaptive_filtration_s", rp.adaptive_filtration_s) && get("filter", rp.filter) &&
-            get("precision", rp.precision);
+            get("precision", rp.precision) && get("anode_angle", rp.anode_angle);
   if (!ok) return FCT_ERR_INVALID_ARGUMENT;
   rp.output_file = config_file + "_recon.dat";        // parse_config.cpp:43
   if (rp.slice_pitch == 0) rp.slice_pitch = rp.slice_thickness;   // parse_config.cpp:59-60
@@ -382,7 +382,7 @@ void fct_make_phantom(uint64_t seed, double z_lo, double z_hi, int n,
 bool fct_synthetic_workload(const std::string& name, uint64_t seed, fct_scan_geometry* g,
                             fct_recon_config* rc, double* i0, std::vector<fct_ellipsoid>* ell,
                             uint64_t* seed_used) {
-  double R = 541.0, Dsd = 949.0, drow_iso = 0.625, cc_off = 0.25, pitch_z = 0.0;
+  double R = 541.0, Dsd = 949.0, drow_iso = 0.625, cc_off = 0.25;
   int nch, nrows, ppr, ffs = 0;
   double dgamma, pitch, rotations, zlo, zhi, fov, thick;
   int nx;
@@ -391,8 +391,8 @@ bool fct_synthetic_workload(const std::string& name, uint64_t seed, fct_scan_geo
   if (name == "ffs") {   // scans.ffs(): Siemens-like, phi + z FFS (definitionas.scanner:5-14)
     R = 595.0; Dsd = 1085.6; drow_iso = 0.6; cc_off = 366.25 - 367.5;
     nch = 736; nrows = 32; ppr = 2304; dgamma = 0.0011844; ffs = FCT_FFS_PHI_Z;
-    pitch = 0.8; rotations = 8; zlo = -50; zhi = 50; fov = 500; thick = 1.0; nx = 512;
-    pitch_z = 0.5; filt = "sharp"; s = 1002; *i0 = 2.5e4;
+    pitch = 0.8; rotations = 8; zlo = -100; zhi = 100; fov = 500; thick = 1.0; nx = 512;
+    filt = "sharp"; s = 1002; *i0 = 2.5e4;
   } else if (name == "small") {
     nch = 736; nrows = 16; ppr = 1152; dgamma = 2.0 * std::asin(250.0 / R) / 735.0;
     pitch = 1.0; rotations = 2; zlo = -20; zhi = 20; fov = 250; thick = 1.25; nx = 256;
@@ -401,9 +401,9 @@ bool fct_synthetic_workload(const std::string& name, uint64_t seed, fct_scan_geo
     nch = 888; nrows = 64; ppr = 984; dgamma = 0.0010831;
     pitch = 0.984; rotations = 10; zlo = -100; zhi = 100; fov = 500; thick = 0.625; nx = 512;
     filt = "ramp"; s = 1001; *i0 = 5e4;
-  } else if (name == "adjoint") {
-    nch = 888; nrows = 64; ppr = 984; dgamma = 0.0010831;
-    pitch = 0.984; rotations = 3; zlo = -20; zhi = 20; fov = 500; thick = 0.625; nx = 512;
+  } else if (name == "adjoint") {   // scans.adjoint(): the small detector at 984 views/rot
+    nch = 736; nrows = 16; ppr = 984; dgamma = 2.0 * std::asin(250.0 / R) / 735.0;
+    pitch = 1.0; rotations = 3; zlo = -20; zhi = 20; fov = 500; thick = 0.625; nx = 512;
     filt = "ramp"; s = 1004; *i0 = 0.0;
   } else {
     return false;
@@ -432,7 +432,7 @@ bool fct_synthetic_workload(const std::string& name, uint64_t seed, fct_scan_geo
   rc->end_pos = zhi;
   rc->recon_fov = fov;
   rc->slice_thickness = thick;
-  rc->slice_pitch = pitch_z > 0.0 ? pitch_z : thick;
+  rc->slice_pitch = thick;
   rc->nx = rc->ny = nx;
   FilterSpec f{};
   filter_spec(filt, &f);
@@ -561,6 +561,72 @@ int ReconstructionMachine::LoadRawData() {
   return st;
 }
 
+// The reference reads every view's tube angle and table position into
+// tables (ReconstructionMachine.cu:96-102) that its backprojection indexes
+// per view (cuda_kernels.cuh:230,235); its rebinning assumes uniform views
+// (cuda_kernels.cuh:15-19).  The plan takes the law
+// a_v = angle0 + rotation*v*2pi/ppr, z_v = z0 + v*feed/ppr: this derives it
+// from views 0, 1 and n-1 and checks it against EVERY view, within 2 % of a
+// view step in angle and 2 % of the collimated slice width in z (plus two
+// float32 ulps of the stored value).  A scan off that law (jittered or
+// non-uniform angles or table positions, a step that is not +-2pi/ppr) is
+// rejected with FCT_ERR_UNSUPPORTED rather than reconstructed wrongly.
+int ReconstructionMachine::ViewLaw(const RawDataSet& ds, fct_scan_geometry* g, std::string* err) {
+  constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
+  const int n = g->n_views, ppr = g->views_per_rotation;
+  if (n < 1 || ppr < 1) {
+    *err = "scan has no views or no projections per rotation";
+    return FCT_ERR_INVALID_ARGUMENT;
+  }
+  for (int v = 0; v < n; ++v)
+    if (!ds.projectionIsAvailable(v)) {
+      *err = "projection " + std::to_string(v) + " is not loaded";
+      return FCT_ERR_INVALID_ARGUMENT;
+    }
+  const double dth = kTwoPi / ppr;
+  g->angle0 = ds.getTubeAngle(0);
+  g->z0 = ds.getTablePosition(0);
+  g->rotation = 1;
+  if (n > 1) {
+    const double step = std::remainder((double)ds.getTubeAngle(1) - g->angle0, kTwoPi);
+    if (std::fabs(step - dth) <= 0.25 * dth) {
+      g->rotation = 1;
+    } else if (std::fabs(step + dth) <= 0.25 * dth) {
+      g->rotation = -1;
+    } else {
+      char b[160];
+      std::snprintf(b, sizeof b, "tube angle step %.6g rad between views 0 and 1 is not "
+                    "+-2pi/%d", step, ppr);
+      *err = b;
+      return FCT_ERR_UNSUPPORTED;
+    }
+    g->feed = ((double)ds.getTablePosition(n - 1) - g->z0) * ppr / (n - 1);
+  }
+  const double csw = g->r * g->row_spacing / g->dsd;
+  auto ulp2 = [](float x) { return 2.0 * (double)std::nextafter(std::fabs(x), INFINITY) -
+                                   2.0 * (double)std::fabs(x); };
+  for (int v = 0; v < n; ++v) {
+    const float av = ds.getTubeAngle(v), zv = ds.getTablePosition(v);
+    const double da = std::remainder((double)av - (g->angle0 + g->rotation * v * dth), kTwoPi);
+    const double dz = (double)zv - (g->z0 + v * g->feed / ppr);
+    const double ta = 0.02 * dth + ulp2(av) + ulp2((float)g->angle0);
+    const double tz = 0.02 * csw + ulp2(zv) + ulp2((float)g->z0);
+    if (!(std::fabs(da) <= ta) || !(std::fabs(dz) <= tz)) {
+      char b[256];
+      std::snprintf(b, sizeof b, "view %d is off the uniform helical law: angle off by %.3g rad "
+                    "(tolerance %.3g), table position off by %.3g mm (tolerance %.3g)",
+                    v, da, ta, dz, tz);
+      *err = b;
+      return FCT_ERR_UNSUPPORTED;
+    }
+  }
+  if (n > 1 && !(std::fabs(g->feed) > 0)) {
+    *err = "table position does not move (axial scans are not supported)";
+    return FCT_ERR_UNSUPPORTED;
+  }
+  return FCT_OK;
+}
+
 int ReconstructionMachine::ConfigureCTGeometry() {
   if (!m_org_data_set) {
     m_err = "no raw data loaded";
@@ -582,22 +648,20 @@ int ReconstructionMachine::ConfigureCTGeometry() {
     g.r = ds.getDistSourceToIsocenter();
     g.central_channel = ds.getDetectorCentralChannel();
     g.central_row = ds.getDetectorCentralRow();
-    g.angle0 = ds.getTubeAngle(0);
-    g.z0 = ds.getTablePosition(0);
     // FlyingFocalSpotMode (DicomDataSet.cpp:193, 285): the DICOM-CT-PD values
     // FFSNONE / FFSXY / FFSZ / FFSXYZ.  The anode angle is not part of
-    // RawDataSet; 7 degrees, the Siemens value, is assumed.
+    // RawDataSet: the recon key anode_angle (rad), else 7 degrees, the
+    // Siemens value.
     const std::string fm = ds.getFlyingFocalSpotMode();
     g.ffs_mode = fm == "FFSXYZ" ? FCT_FFS_PHI_Z
                  : fm == "FFSXY" ? FCT_FFS_PHI
                  : fm == "FFSZ"  ? FCT_FFS_Z
                                  : FCT_FFS_NONE;
-    g.anode_angle = g.ffs_mode ? 7.0 * 3.14159265358979323846 / 180.0 : 0.0;
-    if (g.n_views > g.views_per_rotation)
-      g.feed = (double)ds.getTablePosition(g.views_per_rotation) - g.z0;
-    else if (g.n_views > 1)
-      g.feed = ((double)ds.getTablePosition(g.n_views - 1) - g.z0) * g.views_per_rotation /
-               (g.n_views - 1);
+    g.anode_angle = !g.ffs_mode ? 0.0
+                    : m_rp.anode_angle > 0 ? (double)m_rp.anode_angle
+                                           : 7.0 * 3.14159265358979323846 / 180.0;
+    const int st = ViewLaw(ds, &g, &m_err);
+    if (st) return st;
   }
   FilterSpec f{};
   if (!filter_spec(m_rp.filter, &f)) {

@@ -36,6 +36,7 @@ struct ReconConfig {
   // Keys of this engine (absent keys keep these defaults).
   std::string filter = "ramp";     // ramp | sharp | medium | smooth | hann | hamming | shepp-logan
   std::string precision = "fp32";  // fp32 | fp16 | bf16
+  float anode_angle = 0;           // rad, z-FFS deflection; 0 -> 7 degrees
 };
 
 // Flat "key: value" YAML maps (the only YAML the path reads or writes).
@@ -192,6 +193,9 @@ class ReconstructionMachine {
   int ConfigureCTGeometry();
   int RunReconstruction();
   void PrintCTGeometry() const;
+  // The uniform helical law of a scan's per-view angles and table positions,
+  // checked against every view (FCT_ERR_UNSUPPORTED when a view is off it).
+  static int ViewLaw(const RawDataSet& ds, fct_scan_geometry* g, std::string* err);
   int SaveReconstruction(const std::string& filepath = "");
 
   void setDevice(int device) { m_device = device; }

@@ -100,11 +100,11 @@ int main(int argc, char** argv) {
                 "\"slice_thickness\": %.9g, \"slice_pitch\": %.9g, \"nx\": %zu, \"ny\": %zu, "
                 "\"recon_kernel\": %d, \"x_origin\": %.9g, \"y_origin\": %.9g, "
                 "\"tube_angle_offset\": %.9g, \"adaptive_filtration_s\": %.9g, "
-                "\"filter\": \"%s\", \"precision\": \"%s\"}\n",
+                "\"filter\": \"%s\", \"precision\": \"%s\", \"anode_angle\": %.9g}\n",
                 rp.raw_data_dir.c_str(), rp.output_dir.c_str(), rp.output_file.c_str(),
                 rp.start_pos, rp.end_pos, rp.recon_fov, rp.slice_thickness, rp.slice_pitch, rp.nx,
                 rp.ny, rp.recon_kernel, rp.x_origin, rp.y_origin, rp.tube_angle_offset,
-                rp.adaptive_filtration_s, rp.filter.c_str(), rp.precision.c_str());
+                rp.adaptive_filtration_s, rp.filter.c_str(), rp.precision.c_str(), rp.anode_angle);
     return 0;
   }
 
@@ -126,10 +126,12 @@ int main(int argc, char** argv) {
     std::printf("{\"n_views\": %d, \"n_rows\": %d, \"n_channels\": %d, \"ppr\": %d, "
                 "\"angle0\": %.9g, \"z0\": %.9g, \"feed\": %.9g, \"dsd\": %.9g, \"r\": %.9g, "
                 "\"channel_spacing\": %.9g, \"row_spacing\": %.9g, \"central_channel\": %.9g, "
-                "\"central_row\": %.9g, \"sum\": %.17g, \"angle_last\": %.9g, \"z_last\": %.9g}\n",
+                "\"central_row\": %.9g, \"sum\": %.17g, \"angle_last\": %.9g, \"z_last\": %.9g, "
+                "\"rotation\": %d, \"ffs_mode\": %d, \"anode_angle\": %.9g}\n",
                 g.n_views, g.n_rows, g.n_channels, g.views_per_rotation, g.angle0, g.z0, g.feed,
                 g.dsd, g.r, g.channel_spacing, g.row_spacing, g.central_channel, g.central_row,
-                sum, ds->getTubeAngle(g.n_views - 1), ds->getTablePosition(g.n_views - 1));
+                sum, ds->getTubeAngle(g.n_views - 1), ds->getTablePosition(g.n_views - 1),
+                g.rotation, g.ffs_mode, g.anode_angle);
     return 0;
   }
 

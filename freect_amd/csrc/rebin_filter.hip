@@ -31,6 +31,10 @@ namespace {
 
 constexpr int kRfThreads = 256;
 
+// A non-finite raw measurement (a dead or saturated detector reading) enters
+// the rebinning as 0, so the filter cannot spread it over its line.
+__device__ __forceinline__ float finite_or_zero(float v) { return __builtin_isfinite(v) ? v : 0.0f; }
+
 __device__ __forceinline__ float rebin_sample(const float* __restrict__ raw, int j, int q,
                                               const RebinDev& e, const RebinFilterArgs& a) {
   // Bilinear at (projection j + aoff + fa of the entry's focal state, fan
@@ -46,8 +50,8 @@ __device__ __forceinline__ float rebin_sample(const float* __restrict__ raw, int
   const int row = q >> a.zshift;
   const float* ra = raw + ((size_t)(ma * a.states + rho) * a.raw_nrows + row) * a.raw_ncols;
   const float* rb = raw + ((size_t)(mb * a.states + rho) * a.raw_nrows + row) * a.raw_ncols;
-  const float a0 = ra[e.b0], a1 = ra[e.b1];
-  const float c0 = rb[e.b0], c1 = rb[e.b1];
+  const float a0 = finite_or_zero(ra[e.b0]), a1 = finite_or_zero(ra[e.b1]);
+  const float c0 = finite_or_zero(rb[e.b0]), c1 = finite_or_zero(rb[e.b1]);
   const float lo = fmaf(e.fa, c0 - a0, a0);
   const float hi = fmaf(e.fa, c1 - a1, a1);
   return fmaf(e.fb, hi - lo, lo);
@@ -181,6 +185,12 @@ __global__ void scale_reset_kernel(float* scale) {
   if (threadIdx.x == 0) reinterpret_cast<uint32_t*>(scale)[1] = 0u;
 }
 
+// |v| as ordered bits, 0 for a non-finite v (the rebinning reads those as 0).
+__device__ __forceinline__ uint32_t abs_bits(float v) {
+  const uint32_t u = __float_as_uint(fabsf(v));
+  return u < 0x7f800000u ? u : 0u;
+}
+
 template <bool kVec>
 __global__ __launch_bounds__(256) void absmax_kernel(const float* __restrict__ x, int64_t n,
                                                      float* scale) {
@@ -191,8 +201,7 @@ __global__ __launch_bounds__(256) void absmax_kernel(const float* __restrict__ x
     const float4* x4 = reinterpret_cast<const float4*>(x);
     const int64_t n4 = n >> 2;
     auto absmax4 = [](const float4 v) {
-      return max(max(__float_as_uint(fabsf(v.x)), __float_as_uint(fabsf(v.y))),
-                 max(__float_as_uint(fabsf(v.z)), __float_as_uint(fabsf(v.w))));
+      return max(max(abs_bits(v.x), abs_bits(v.y)), max(abs_bits(v.z), abs_bits(v.w)));
     };
     for (; i + 3 * stride < n4; i += 4 * stride) {   // four loads in flight per thread
       const float4 v0 = x4[i], v1 = x4[i + stride], v2 = x4[i + 2 * stride],
@@ -202,7 +211,7 @@ __global__ __launch_bounds__(256) void absmax_kernel(const float* __restrict__ x
     for (; i < n4; i += stride) m = max(m, absmax4(x4[i]));
     i = 4 * n4 + (int64_t)blockIdx.x * 256 + threadIdx.x;
   }
-  for (; i < n; i += stride) m = max(m, __float_as_uint(fabsf(x[i])));
+  for (; i < n; i += stride) m = max(m, abs_bits(x[i]));
   for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
   if ((threadIdx.x & 63) == 0 && m) atomicMax(reinterpret_cast<uint32_t*>(scale) + 1, m);
 }

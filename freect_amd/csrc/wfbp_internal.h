@@ -116,6 +116,7 @@ struct PhantomArgs {
   double cyl_r, cyl_mu, i0;
   unsigned long long seed;
   int32_t states;            // focal states (1: no FFS)
+  int32_t mirror;            // 1: clockwise scan, rays mirrored y -> -y
   double ffs_off[4][3];      // per state (dr, da, dz) along (e_w, e_u, e_z)
 };
 

@@ -14,9 +14,11 @@ Workloads (named by what they are):
             slices at 1.25 mm, FOV 250, Shepp-Logan, noiseless.
   full      GE-like 888x64, 984 views/rot, pitch 0.984, 10 rotations (9840
             views), 512^2 x 320 slices at 0.625 mm, FOV 500, ramp, I0 = 5e4.
-  ffs       Siemens-like 736x32 with combined phi/z flying focal spot.
-  batch     24 seeded uneven jobs on the 888x64 detector.
-  adjoint   888x64 detector, 3 rotations (2952 views), 512^2 x 64 slices.
+  ffs       Siemens-like 736x32 with combined phi/z flying focal spot, 512^2 x
+            200 slices of 1.0 mm over 200 mm (partly outside the helix).
+  batch     24 seeded uneven jobs on the 736x16 detector of `small`.
+  adjoint   736x16 detector of `small` at 984 views/rot, pitch 1.0, 3
+            rotations (2952 views), 512^2 x 64 slices at 0.625 mm.
 """
 from __future__ import annotations
 
@@ -66,6 +68,10 @@ class ScanGeometry:
     feed: float = 0.0           # table feed per rotation (mm, signed)
     ffs_mode: int = 0           # 0 none, 1 phi, 2 z, 3 phi+z (focal state = view mod S)
     anode_angle: float = 0.0    # rad (z-FFS deflection)
+    # +1 counter-clockwise (tube angle grows with the view), -1 clockwise: the
+    # mirror image y -> -y of the counter-clockwise scan with angles -a_v
+    # (channel fan angles and phi-FFS deflections run with the rotation).
+    rotation: int = 1
 
     @property
     def ffs_states(self):
@@ -101,7 +107,8 @@ class ScanGeometry:
         return self.z0 + v * self.feed / self.views_per_rotation
 
     def angle_of_view(self, v):
-        return self.angle0 + v * 2.0 * math.pi / self.views_per_rotation
+        sgn = -1.0 if self.rotation < 0 else 1.0
+        return self.angle0 + sgn * v * 2.0 * math.pi / self.views_per_rotation
 
     def replace(self, **kw):
         return dataclasses.replace(self, **kw)
@@ -290,7 +297,13 @@ def full(seed: int = 1001) -> Workload:
 
 
 def adjoint(seed: int = 1004) -> Workload:
-    scan = helical(ge_full_detector(), pitch=0.984, rotations=3)
+    """The small scan's 736 x 16 detector at 984 views per rotation (3
+    rotations = 2952 views), pitch 1.0, noiseless; 512^2 x 64 native slices
+    of 0.625 mm (40 mm, of which the 30 mm of table travel covers the
+    central part fully)."""
+    det = ge_small_detector()
+    det["views_per_rotation"] = 984
+    scan = helical(det, pitch=1.0, rotations=3)
     rc = ReconConfig(start_pos=-20.0, end_pos=20.0, recon_fov=500.0, slice_thickness=0.625,
                      slice_pitch=0.625, nx=512, ny=512, filter="ramp")
     return Workload("adjoint", scan, rc, make_phantom(seed, -20.0, 20.0), 0.0, seed)
@@ -301,22 +314,26 @@ def ffs(seed: int = 1002) -> Workload:
     rows (0.6 mm at iso), 2304 projections per rotation in a 4-state focal
     cycle (interleaved to 1472 channels x 64 rows, 576 parallel views per
     rotation), pitch 0.8, 8 rotations, Poisson noise at I0 = 2.5e4; 512^2 x 200
-    slices of 1.0 mm at 0.5 mm spacing (the 100 mm the 123 mm of table travel
-    covers fully), sharp kernel (c = 1, a = 1)."""
+    slices of 1.0 mm at 1.0 mm spacing over z in [-100, 100) mm, of which the
+    123 mm of table travel covers the central part (HU metrics on the covered
+    mask), sharp kernel (c = 1, a = 1)."""
     det = siemens_detector()
     det["views_per_rotation"] = 2304
     scan = helical(det, pitch=0.8, rotations=8).replace(ffs_mode=3, anode_angle=0.1222)
-    rc = ReconConfig(start_pos=-50.0, end_pos=50.0, recon_fov=500.0, slice_thickness=1.0,
-                     slice_pitch=0.5, nx=512, ny=512, filter="sharp")
-    return Workload("ffs", scan, rc, make_phantom(seed, -50.0, 50.0), 2.5e4, seed)
+    rc = ReconConfig(start_pos=-100.0, end_pos=100.0, recon_fov=500.0, slice_thickness=1.0,
+                     slice_pitch=1.0, nx=512, ny=512, filter="sharp")
+    return Workload("ffs", scan, rc, make_phantom(seed, -100.0, 100.0), 2.5e4, seed)
 
 
 def batch_jobs(seed: int = 1003, n_jobs: int = 24) -> List[Workload]:
-    """Uneven jobs on the 888x64 detector: scan lengths 2-20 rotations, pitch
-    in {0.516, 0.984, 1.375}, matrix in {256^2, 512^2}, 16-240 slices of
-    {0.625, 1.25, 2.5} mm, filters {ramp, shepp-logan, hann}, I0 in [1e4, 1e5]."""
+    """Uneven jobs on the 736x16 detector of `small` (1152 views/rot): scan
+    lengths 2-20 rotations, pitch in {0.516, 0.984, 1.375}, matrix in
+    {256^2, 512^2}, 16-240 slices of {0.625, 1.25, 2.5} mm, filters {ramp,
+    shepp-logan, hann}, I0 in [1e4, 1e5].  The slice count drawn is cut to the
+    z-range the helix covers fully (at least 16 slices, which may reach past
+    it on the shortest scans)."""
     rng = SplitMix64(seed)
-    det = ge_full_detector()
+    det = ge_small_detector()
     jobs = []
     for j in range(n_jobs):
         rot = rng.randint(2, 20)

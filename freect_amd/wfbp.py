@@ -33,7 +33,7 @@ class ScanGeometryC(C.Structure):
                 ("row_spacing", C.c_double), ("dsd", C.c_double), ("r", C.c_double),
                 ("central_channel", C.c_double), ("central_row", C.c_double),
                 ("angle0", C.c_double), ("z0", C.c_double), ("feed", C.c_double),
-                ("ffs_mode", C.c_int32), ("ffs_reserved", C.c_int32),
+                ("ffs_mode", C.c_int32), ("rotation", C.c_int32),
                 ("anode_angle", C.c_double)]
 
 
@@ -137,7 +137,7 @@ def scan_struct(scan: S.ScanGeometry) -> ScanGeometryC:
     return ScanGeometryC(scan.n_channels, scan.n_rows, scan.n_views, scan.views_per_rotation,
                          scan.channel_spacing, scan.row_spacing, scan.dsd, scan.r,
                          scan.central_channel, scan.central_row, scan.angle0, scan.z0, scan.feed,
-                         scan.ffs_mode, 0, scan.anode_angle)
+                         scan.ffs_mode, scan.rotation, scan.anode_angle)
 
 
 def recon_struct(rc: S.ReconConfig) -> ReconConfigC:

@@ -41,9 +41,11 @@ typedef enum {
 } fct_precision;
 
 /* Scan description: what RawDataSet.h:91-121 exposes, with the per-view tube
- * angle and table position as the linear law angle0 + v*2pi/ppr and
+ * angle and table position as the linear law angle0 + rotation*v*2pi/ppr and
  * z0 + v*feed/ppr (the reference's rebinning already assumes uniform views,
- * cuda_kernels.cuh:15-19). */
+ * cuda_kernels.cuh:15-19; its backprojection reads the per-view tables,
+ * ReconstructionMachine.cu:96-102, which the C++ host checks against this law
+ * before it builds a plan). */
 typedef struct {
   int32_t n_channels;          /* getDetectorChannels */
   int32_t n_rows;              /* getDetectorRows */
@@ -65,7 +67,12 @@ typedef struct {
    * (fct_plan_info.n_filtered_*): phi doubles the channels, z the rows, and
    * there are n_views / S parallel views. */
   int32_t ffs_mode;            /* fct_ffs_mode */
-  int32_t ffs_reserved;
+  /* Direction of the tube's rotation: +1 (or 0) counter-clockwise, the tube
+   * angle growing with the view; -1 clockwise.  The fan angle of a channel,
+   * (c - central_channel) * channel_spacing / dsd, and the phi-FFS
+   * deflection are measured in the direction of rotation, so a clockwise
+   * scan is the mirror image (y -> -y) of a counter-clockwise one. */
+  int32_t rotation;
   double anode_angle;          /* rad; z-FFS deflection dr tan(anode) */
 } fct_scan_geometry;
 

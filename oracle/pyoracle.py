@@ -23,7 +23,7 @@ class OrScan(C.Structure):
                 ("dsd", C.c_double), ("r", C.c_double),
                 ("central_channel", C.c_double), ("central_row", C.c_double),
                 ("angle0", C.c_double), ("z0", C.c_double), ("feed", C.c_double),
-                ("ffs_mode", C.c_int32), ("ffs_reserved", C.c_int32),
+                ("ffs_mode", C.c_int32), ("rotation", C.c_int32),
                 ("anode_angle", C.c_double)]
 
 
@@ -73,6 +73,7 @@ def lib():
             "or_slice_grid": (C.c_int, [C.c_double, C.c_double, C.c_double, dp, C.c_int]),
             "or_filter_kernel": (None, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, dp]),
             "or_rebin": (None, [P(OrScan), fp, dp, C.c_int, C.c_int]),
+            "or_rebin_entries": (C.c_int, [P(OrScan), i32p, dp]),
             "or_filter_rows": (None, [dp, C.c_int64, C.c_int, dp, C.c_int, C.c_int]),
             "or_backproject": (C.c_int, [P(OrScan), P(OrRecon), dp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, dp, i64p]),
@@ -110,7 +111,29 @@ def scan_struct(scan) -> OrScan:
     return OrScan(scan.n_channels, scan.n_rows, scan.n_views, scan.views_per_rotation,
                   scan.channel_spacing, scan.row_spacing, scan.dsd, scan.r,
                   scan.central_channel, scan.central_row, scan.angle0, scan.z0, scan.feed,
-                  getattr(scan, "ffs_mode", 0), 0, getattr(scan, "anode_angle", 0.0))
+                  getattr(scan, "ffs_mode", 0), getattr(scan, "rotation", 1),
+                  getattr(scan, "anode_angle", 0.0))
+
+
+def _cw(scan) -> bool:
+    return getattr(scan, "rotation", 1) < 0
+
+
+def _mirror(scan, rc):
+    """A clockwise scan is the mirror image y -> -y of the counter-clockwise
+    scan with tube angles -a_v: reconstruct that one on the grid mirrored in
+    y (y_origin -> -y_origin), whose row ny-1-i is row i of the clockwise
+    grid.  Returns the counter-clockwise (scan, recon) and whether the rows
+    of the result must be reversed."""
+    if not _cw(scan):
+        return scan, rc, False
+    ms = scan.replace(angle0=-scan.angle0, rotation=1)
+    mr = rc.replace(y_origin=-rc.y_origin) if rc is not None else None
+    return ms, mr, True
+
+
+def _flip_rows(a, flip):
+    return np.ascontiguousarray(a[..., ::-1, :]) if flip else a
 
 
 def _eff(scan):
@@ -178,6 +201,17 @@ def rebin(scan, raw: np.ndarray, v0=0, v1=None) -> np.ndarray:
     return out
 
 
+def rebin_entries(scan):
+    """The rebinning table of the scan (one row parity, two with z FFS):
+    int fields (valid, rho, b0, b1, aoff, mmax) and FP64 (fb, fa) per entry."""
+    s = scan_struct(scan)
+    n = lib().or_rebin_entries(C.byref(s), None, None)
+    ient = np.zeros((n, 6), np.int32)
+    fent = np.zeros((n, 2), np.float64)
+    lib().or_rebin_entries(C.byref(s), _ptr(ient, C.c_int32), _ptr(fent, C.c_double))
+    return ient, fent
+
+
 def filter_rows(rows: np.ndarray, h: np.ndarray, nfft: int, use_fft=True) -> np.ndarray:
     out = np.ascontiguousarray(rows, np.float64).copy()
     ncols = out.shape[-1]
@@ -188,6 +222,12 @@ def filter_rows(rows: np.ndarray, h: np.ndarray, nfft: int, use_fft=True) -> np.
 
 
 def backproject(scan, rc, filtered: np.ndarray, v0: int, s0: int, s1: int):
+    scan, rc, flip = _mirror(scan, rc)
+    vol, n = _backproject(scan, rc, filtered, v0, s0, s1)
+    return _flip_rows(vol, flip), n
+
+
+def _backproject(scan, rc, filtered: np.ndarray, v0: int, s0: int, s1: int):
     filtered = np.ascontiguousarray(filtered, np.float64)
     v1 = v0 + filtered.shape[0]
     vol = np.zeros((s1 - s0, rc.ny, rc.nx), np.float64)
@@ -201,6 +241,7 @@ def backproject(scan, rc, filtered: np.ndarray, v0: int, s0: int, s1: int):
 
 
 def backproject_f32(scan, rc, filtered: np.ndarray, v0: int, s0: int, s1: int):
+    scan, rc, flip = _mirror(scan, rc)
     filtered = np.ascontiguousarray(filtered, np.float32)
     v1 = v0 + filtered.shape[0]
     vol = np.zeros((s1 - s0, rc.ny, rc.nx), np.float32)
@@ -210,7 +251,7 @@ def backproject_f32(scan, rc, filtered: np.ndarray, v0: int, s0: int, s1: int):
                                    v1, s0, s1, _ptr(vol, C.c_float), C.byref(n))
     if rc_ != 0:
         raise RuntimeError(f"or_backproject_f32 failed ({rc_})")
-    return vol, n.value
+    return _flip_rows(vol, flip), n.value
 
 
 def thicken(scan, rc, native: np.ndarray) -> np.ndarray:
@@ -223,19 +264,21 @@ def thicken(scan, rc, native: np.ndarray) -> np.ndarray:
 
 
 def bp_counts(scan, rc, s0, s1):
+    scan, rc, flip = _mirror(scan, rc)
     counts = np.zeros((s1 - s0, rc.ny, rc.nx), np.int32)
     n = C.c_int64(0)
     s, r = scan_struct(_eff(scan)), recon_struct(rc)
     lib().or_bp_counts(C.byref(s), C.byref(r), s0, s1, _ptr(counts, C.c_int32), C.byref(n))
-    return counts, n.value
+    return _flip_rows(counts, flip), n.value
 
 
 def bp_coverage(scan, rc, s0, s1) -> np.ndarray:
     """Per native voxel: min over theta~ of the weight sum of selected views."""
+    scan, rc, flip = _mirror(scan, rc)
     out = np.zeros((s1 - s0, rc.ny, rc.nx), np.float64)
     s, r = scan_struct(_eff(scan)), recon_struct(rc)
     lib().or_bp_coverage(C.byref(s), C.byref(r), s0, s1, _ptr(out, C.c_double))
-    return out
+    return _flip_rows(out, flip)
 
 
 def covered_mask(scan, rc, threshold=0.5) -> np.ndarray:
@@ -254,6 +297,9 @@ def covered_mask(scan, rc, threshold=0.5) -> np.ndarray:
 
 
 def bp_selection(scan, rc, ix, iy, iz) -> np.ndarray:
+    scan, rc, flip = _mirror(scan, rc)
+    if flip:
+        iy = rc.ny - 1 - iy
     sel = np.zeros(_eff(scan).n_views, np.int32)
     s, r = scan_struct(_eff(scan)), recon_struct(rc)
     lib().or_bp_selection(C.byref(s), C.byref(r), ix, iy, iz, _ptr(sel, C.c_int32))
@@ -288,6 +334,7 @@ def view_range(scan, rc, s0, s1):
 
 
 def reconstruct_native(scan, rc, raw: np.ndarray, s0: int, s1: int):
+    scan, rc, flip = _mirror(scan, rc)
     raw = np.ascontiguousarray(raw, np.float32)
     vol = np.zeros((s1 - s0, rc.ny, rc.nx), np.float64)
     n = C.c_int64(0)
@@ -296,11 +343,12 @@ def reconstruct_native(scan, rc, raw: np.ndarray, s0: int, s1: int):
                                       _ptr(vol, C.c_double), C.byref(n))
     if rc_ != 0:
         raise RuntimeError(f"or_reconstruct_native failed ({rc_})")
-    return vol, n.value
+    return _flip_rows(vol, flip), n.value
 
 
 def backproject_adjoint(scan, rc, xbar: np.ndarray, v0: int, v1: int, s0: int) -> np.ndarray:
-    xbar = np.ascontiguousarray(xbar, np.float64)
+    scan, rc, flip = _mirror(scan, rc)
+    xbar = np.ascontiguousarray(_flip_rows(np.asarray(xbar), flip), np.float64)
     s1 = s0 + xbar.shape[0]
     e = _eff(scan)
     g = np.zeros((v1 - v0, e.n_rows, e.n_channels), np.float64)
@@ -322,7 +370,8 @@ def rebin_adjoint(scan, gbar: np.ndarray, v0: int = 0) -> np.ndarray:
 
 
 def adjoint_native(scan, rc, xbar: np.ndarray, s0: int = 0) -> np.ndarray:
-    xbar = np.ascontiguousarray(xbar, np.float64)
+    scan, rc, flip = _mirror(scan, rc)
+    xbar = np.ascontiguousarray(_flip_rows(np.asarray(xbar), flip), np.float64)
     s1 = s0 + xbar.shape[0]
     p = np.zeros((scan.n_views, scan.n_rows, scan.n_channels), np.float64)
     s, r = scan_struct(scan), recon_struct(rc)

@@ -140,6 +140,7 @@ int bp_core(const or_scan* s, const or_recon* r, const P* filtered, int v0,
   const int nn = (int)sg.zn.size();
   if (s0 < 0 || s1 > nn || s0 >= s1) return -1;
   if (G.ppr % 2 != 0) return -2;
+  if (s->rotation < 0) return -3;   // counter-clockwise only (pyoracle mirrors)
   const int ns = s1 - s0;
   const int nx = r->nx, ny = r->ny;
   const int H = G.H;
@@ -342,11 +343,32 @@ void or_rebin(const or_scan* s, const float* raw, double* out, int v0, int v1) {
         const int mb = std::min(std::max(j + e.aoff + 1, 0), e.mmax);
         const float* ra = raw + (size_t)(ma * F.S + e.rho) * rvs + (size_t)row * F.ncols;
         const float* rb = raw + (size_t)(mb * F.S + e.rho) * rvs + (size_t)row * F.ncols;
-        o[k] = (1.0 - e.fb) * ((1.0 - e.fa) * ra[e.b0] + e.fa * rb[e.b0]) +
-               e.fb * ((1.0 - e.fa) * ra[e.b1] + e.fa * rb[e.b1]);
+        // A non-finite raw value (dead or saturated reading) counts as 0.
+        auto fin = [](float v) { return std::isfinite(v) ? (double)v : 0.0; };
+        o[k] = (1.0 - e.fb) * ((1.0 - e.fa) * fin(ra[e.b0]) + e.fa * fin(rb[e.b0])) +
+               e.fb * ((1.0 - e.fa) * fin(ra[e.b1]) + e.fa * fin(rb[e.b1]));
       }
     }
   }
+}
+
+int or_rebin_entries(const or_scan* s, int32_t* ient, double* fent) {
+  // The rebinning table as the plan builds it (fct_ffs::rebin_entry per
+  // effective channel and row parity): ient[e*6 + {valid, rho, b0, b1, aoff,
+  // mmax}], fent[e*2 + {fb, fa}], e = zb*e_ncols + k.  Returns the entries.
+  const fct_ffs::Geometry F = ffs_geometry(s);
+  const int zpar = fct_ffs::has_z(F.mode) ? 2 : 1;
+  if (!ient || !fent) return zpar * F.e_ncols;
+  for (int zb = 0; zb < zpar; ++zb)
+    for (int k = 0; k < F.e_ncols; ++k) {
+      const fct_ffs::Entry e = fct_ffs::rebin_entry(F, k, zb);
+      const size_t i = (size_t)zb * F.e_ncols + k;
+      const int32_t v[6] = {e.valid, e.rho, e.b0, e.b1, e.aoff, e.mmax};
+      for (int m = 0; m < 6; ++m) ient[i * 6 + m] = v[m];
+      fent[i * 2] = e.fb;
+      fent[i * 2 + 1] = e.fa;
+    }
+  return zpar * F.e_ncols;
 }
 
 void or_filter_rows(double* rows, int64_t nlines, int ncols, const double* h,
@@ -447,6 +469,7 @@ int or_thicken(const or_scan* s, const or_recon* r, const double* native, double
 
 int or_bp_selection(const or_scan* s, const or_recon* r, int ix, int iy, int is,
                     int32_t* sel) {
+  if (s->rotation < 0) return -3;   // counter-clockwise only (pyoracle mirrors)
   const Geo G = make_geo(s);
   const SliceGrid sg = make_slices(G, r);
   const int chunk = fct_tables::kSliceChunk;
@@ -477,22 +500,26 @@ void or_project_phantom(const or_scan* s, double cyl_radius, double cyl_mu,
   const Geo G = make_geo(s);
   const fct_ffs::Geometry F = ffs_geometry(s);
   const size_t vs = (size_t)G.nrows * G.ncols;
+  // Direction of rotation: the tube angle, the fan angle of a channel and
+  // the tangential unit vector e_u all run with it (sg = -1: clockwise).
+  const double sg = s->rotation < 0 ? -1.0 : 1.0;
 #pragma omp parallel for schedule(static)
   for (int v = v0; v < v1; ++v) {
-    const double a = s->angle0 + G.dtheta * (double)v;
+    const double a = s->angle0 + sg * G.dtheta * (double)v;
     // Flying focal spot: state v mod S moves the focal spot by (dr, da, dz)
-    // along (e_w, e_u, e_z); the detector stays where the undeflected source
-    // puts it (generate_system_matrix_ffs.cpp:126-149).
+    // along (e_w, e_u, e_z), e_w = (cos a, sin a), e_u = sg (-sin a, cos a);
+    // the detector stays where the undeflected source puts it
+    // (generate_system_matrix_ffs.cpp:126-149).
     const fct_ffs::Deflection fd =
         fct_ffs::state_offset(F.mode, v % F.S, F.dr, F.da, F.anode);
-    const double ox = fd.dr * std::cos(a) - fd.da * std::sin(a);
-    const double oy = fd.dr * std::sin(a) + fd.da * std::cos(a);
+    const double ox = fd.dr * std::cos(a) - sg * fd.da * std::sin(a);
+    const double oy = fd.dr * std::sin(a) + sg * fd.da * std::cos(a);
     const double sx = s->r * std::cos(a) + ox, sy = s->r * std::sin(a) + oy;
     const double sz = s->z0 + G.dz * (double)v + fd.dz;
     for (int row = 0; row < G.nrows; ++row) {
       const double w = ((double)row - s->central_row) * s->row_spacing;
       for (int c = 0; c < G.ncols; ++c) {
-        const double beta = ((double)c - s->central_channel) * G.dgamma;
+        const double beta = sg * ((double)c - s->central_channel) * G.dgamma;
         // Ray from the (deflected) source to its detector cell, unit direction u.
         const double dx = -s->dsd * std::cos(a + beta) - ox;
         const double dy = -s->dsd * std::sin(a + beta) - oy;

@@ -39,7 +39,7 @@ typedef struct {
   double z0;                // source z of view 0 (mm)
   double feed;              // table feed per rotation (mm, signed)
   int32_t ffs_mode;         // 0 none, 1 phi, 2 z, 3 phi+z (focal state = view mod S)
-  int32_t ffs_reserved;
+  int32_t rotation;         // +1 (or 0) counter-clockwise, -1 clockwise
   double anode_angle;       // rad
 } or_scan;
 
@@ -67,6 +67,11 @@ typedef struct {
   double dmu;        // additive attenuation (1/mm)
 } or_ellipsoid;
 
+// A clockwise scan (rotation -1) is the mirror image y -> -y of the
+// counter-clockwise scan with tube angles -a_v: or_project_phantom traces its
+// rays directly; the reconstruction functions take counter-clockwise scans
+// (pyoracle mirrors a clockwise one: angle0 -> -angle0, y_origin -> -y_origin,
+// rows of the result reversed).
 // The interleaved detector an FFS scan is reconstructed on (ffs_mode 0);
 // the scan itself without FFS.  Every function below except or_rebin,
 // or_rebin_adjoint, or_project_phantom, or_reconstruct_native and
@@ -78,6 +83,10 @@ int or_geometry(const or_scan* s, const or_recon* r, or_derived* d);
 int or_slice_grid(double start, double end, double step, double* z, int cap);
 void or_filter_kernel(int n, double ds, int kind, double c, double a, double* h);
 void or_rebin(const or_scan* s, const float* raw, double* out, int v0, int v1);
+// The rebinning table (fct_ffs::rebin_entry, as the plan uploads it):
+// ient[e][valid, rho, b0, b1, aoff, mmax], fent[e][fb, fa]; returns the entry
+// count (call with NULL buffers to size them).
+int or_rebin_entries(const or_scan* s, int32_t* ient, double* fent);
 void or_filter_rows(double* rows, int64_t nlines, int ncols, const double* h,
                     int nfft, int use_fft);
 // filtered: parallel views [v0, v1) of [view][row][ch] FP64.  Computes native

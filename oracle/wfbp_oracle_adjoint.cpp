@@ -45,6 +45,7 @@ int or_backproject_adjoint(const or_scan* s, const or_recon* r, const double* xb
   const int nn = d.n_slices_native;
   if (s0 < 0 || s1 > nn || s0 >= s1) return -1;
   if (s->views_per_rotation % 2) return -2;
+  if (s->rotation < 0) return -3;   // counter-clockwise only (pyoracle mirrors)
   const int ncols = s->n_channels, nrows = s->n_rows, nviews = s->n_views;
   const int H = s->views_per_rotation / 2, ns = s1 - s0, nx = r->nx, ny = r->ny;
   const double dtheta = 2.0 * kPi / s->views_per_rotation, dz = s->feed / s->views_per_rotation;
