@@ -94,47 +94,111 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(w, raw_np, n_vv_total, n_native, n_slices=8, n_rf_views=960):
-    """FP64 CPU restatement on a bounded sample (about 10 s on 16 cores),
-    scaled to the whole job: rebin + filter of n_rf_views views (scaled to all
-    views) + backprojection of n_slices central native slices (scaled by the
-    exact voxel-view count)."""
+def _lscpu():
+    """CPU model, sockets and threads per core (lscpu), for the baseline record."""
+    import subprocess
+    info = {"model": _cpu_model()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)",
+                             "Model name", "CPU max MHz"):
+                info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    return info
+
+
+def cpu_baseline(w, raw_np, n_vv_total, n_native, n_rf_views=960,
+                 slice_groups=((0, 2), (None, 4), (-2, None)), repeats=3):
+    """The CPU restatement on a bounded sample of the full job, FP64 (the
+    oracle) and FP32 (the same templated backprojection loop in float):
+    rebin + filter of n_rf_views central views (scaled to all views) and the
+    backprojection of native slices at both ends and in the middle of the
+    volume (scaled by the exact voxel-view count).  One warm-up, then the
+    median of `repeats` runs of each stage; the spread of the per-voxel-view
+    backprojection time between the three slice groups is the sampling error
+    of the extrapolation.  OpenMP on every host core the process may use
+    (OMP_PROC_BIND=close)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import statistics
     import numpy as np
     import pyoracle
 
     sc, rc = w.scan, w.recon
     d = pyoracle.geometry(sc, rc)
-    cores = os.cpu_count() or 1
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except Exception:
-        pass
-    # (a) rebin + filter sample.
-    va = sc.n_views // 2 - n_rf_views // 2
-    t = time.perf_counter()
-    par = pyoracle.rebin(sc, raw_np, va, va + n_rf_views)
+    cores = _cores()
     h = pyoracle.filter_kernel(d.nfft, d.ds, rc.filter_kind, rc.filter_c, rc.filter_a)
-    par = pyoracle.filter_rows(par, h, d.nfft, use_fft=True)
-    t_rf_sample = time.perf_counter() - t
-    t_rf = t_rf_sample * sc.n_views / float(n_rf_views)
-    # (b) backprojection of central native slices.
-    s0 = n_native // 2 - n_slices // 2
-    v0, v1 = pyoracle.view_range(sc, rc, s0, s0 + n_slices)
-    par = pyoracle.rebin(sc, raw_np, v0, v1)
-    par = pyoracle.filter_rows(par, h, d.nfft, use_fft=True)
-    t = time.perf_counter()
-    vol, n_sel = pyoracle.backproject(sc, rc, par, v0, s0, s0 + n_slices)
-    t_bp = time.perf_counter() - t
-    t_total = t_rf + t_bp * n_vv_total / max(n_sel, 1)
-    return {"value": n_vv_total / t_total, "unit": "voxel-views/s", "cores": cores,
-            "kind": "port",
-            "sample": (f"FP64 OpenMP restatement on {cores} threads ({_cpu_model()}): rebin+filter "
-                       f"of {n_rf_views} of {sc.n_views} views ({t_rf_sample:.2f} s) scaled to all "
-                       f"views, plus backprojection of native slices {s0}..{s0 + n_slices - 1} "
-                       f"({n_sel} voxel-views, {t_bp:.2f} s) scaled by the exact voxel-view count"),
-            "stage_s": {"rebin_filter": t_rf, "backproject": t_bp * n_vv_total / max(n_sel, 1)},
-            "s_per_volume_est": t_total}, (s0, vol)
+    va = sc.n_views // 2 - n_rf_views // 2
+
+    def rf_sample():
+        par = pyoracle.rebin(sc, raw_np, va, va + n_rf_views)
+        return pyoracle.filter_rows(par, h, d.nfft, use_fft=True)
+
+    groups = []
+    for a, n in slice_groups:
+        if a is None:
+            s0 = n_native // 2 - n // 2
+            s1 = s0 + n
+        elif n is None:
+            s0, s1 = n_native + a, n_native
+        else:
+            s0, s1 = a, a + n
+        v0, v1 = pyoracle.view_range(sc, rc, s0, s1)
+        par = pyoracle.filter_rows(pyoracle.rebin(sc, raw_np, v0, v1), h, d.nfft, use_fft=True)
+        groups.append((s0, s1, v0, par))
+
+    def timed(fn):
+        fn()   # warm-up
+        ts = []
+        for _ in range(repeats):
+            t = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t)
+        return statistics.median(ts), ts, out
+
+    t_rf, t_rf_all, _ = timed(rf_sample)
+    res = {}
+    refs = []
+    for prec in ("fp64", "fp32"):
+        per_vv, t_groups, n_groups = [], [], []
+        for gi, (s0, s1, v0, par) in enumerate(groups):
+            if prec == "fp64":
+                fn = lambda: pyoracle.backproject(sc, rc, par, v0, s0, s1)   # noqa: E731
+            else:
+                p32 = par.astype(np.float32)
+                fn = lambda: pyoracle.backproject_f32(sc, rc, p32, v0, s0, s1)   # noqa: E731
+            tm, _, (vol, n_sel) = timed(fn)
+            per_vv.append(tm / max(n_sel, 1))
+            t_groups.append(tm)
+            n_groups.append(n_sel)
+            if prec == "fp64":
+                refs.append((s0, vol))
+        n_s = sum(n_groups)
+        t_bp_vv = sum(t_groups) / max(n_s, 1)
+        t_bp = t_bp_vv * n_vv_total
+        t_rf_total = t_rf * sc.n_views / float(n_rf_views)
+        total = t_rf_total + t_bp
+        res[prec] = {"value": n_vv_total / total, "s_per_volume_est": total,
+                     "stage_s": {"rebin_filter": t_rf_total, "backproject": t_bp},
+                     "bp_s_per_voxel_view_by_group": per_vv,
+                     "extrapolation_spread": (max(per_vv) - min(per_vv)) / t_bp_vv}
+    groups_txt = ", ".join(f"{s0}..{s1 - 1}" for s0, s1, _, _ in groups)
+    out = dict(res["fp64"])
+    out.update({
+        "unit": "voxel-views/s", "cores": cores, "kind": "port",
+        "sample": (f"FP64 OpenMP restatement on {cores} threads ({_cpu_model()}), warm-up + "
+                   f"median of {repeats}: rebin+filter of {n_rf_views} of {sc.n_views} views "
+                   f"({t_rf:.2f} s) scaled to all views, plus backprojection of native slices "
+                   f"{groups_txt} scaled by the exact voxel-view count"),
+        "fp32": res["fp32"],
+        "rebin_filter_runs_s": t_rf_all,
+        "omp": {"OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+                "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")},
+        "lscpu": _lscpu(),
+    })
+    return out, refs
 
 
 def batch_path(ctx, jobs, raws, precision, steps, warmup, torch, candidates=(4, 8, 16)):
@@ -407,10 +471,16 @@ def main():
     ap.add_argument("--workload", default="full", choices=["full", "batch", "adjoint", "ffs"],
                     help="full: the headline scan (default); batch: 24 uneven jobs; adjoint: "
                          "the refinement report")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (HBM traffic, VALU activity)")
+    ap.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--path", default="batch", choices=["batch", "plan"],
                     help="chosen path: hipGraph-replayed Batch with the persistent "
                          "backprojection (default) or the single-plan stage calls")
     args = ap.parse_args()
+    # The CPU baseline's OpenMP threads stay on neighbouring cores (set before
+    # the restatement's OpenMP runtime starts).
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
                                    "(multi-GPU z-slab sharding is design-only)"}))
@@ -423,6 +493,9 @@ def main():
     from freect_amd import wfbp
 
     torch.cuda.set_device(0)
+    if args.probe:
+        traffic_probe(args, torch)
+        return
     if args.copy_mib:
         src = torch.ones(args.copy_mib * 2 ** 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -480,37 +553,78 @@ def main():
     else:
         chosen = res[args.precision]
     info = p0.info
+
+    # End to end from host memory: the scan in pinned host memory, streamed
+    # to the device in chunks of views overlapped with rebin+filter and the
+    # backprojection of the slabs whose views have arrived, the volume copied
+    # back (fct_wfbp_gpu_reconstruct with host buffers).
+    e2e = {}
+    raw_host = raw.cpu().pin_memory()
+    vol_host = torch.empty((info.n_slices_requested, w.recon.ny, w.recon.nx),
+                           dtype=torch.float32).pin_memory()
+    for prec in sorted(plans):
+        plan = plans[prec]
+        for _ in range(max(1, args.warmup)):
+            plan.reconstruct_host(raw_host, out=vol_host.numpy())
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.reconstruct_host(raw_host, out=vol_host.numpy())
+        e2e[prec] = (time.perf_counter() - t0) / args.steps
+        log("end to end", prec, e2e[prec], "s/volume")
+    del raw_host, vol_host
+
     cpu = None
     hu = {}
     if not args.no_cpu:
         log("cpu baseline")
         raw_np = raw.cpu().numpy()
-        cpu, (s0, ref) = cpu_baseline(w, raw_np, n_vv, info.n_slices_native)
+        cpu, refs = cpu_baseline(w, raw_np, n_vv, info.n_slices_native)
+        log("cpu", cpu)
         # HU error of the GPU paths against the FP64 restatement on those
         # slices (native == requested grid for this workload).
         x = (np.arange(w.recon.nx) - 0.5 * (w.recon.nx - 1)) * w.recon.recon_fov / w.recon.nx
         X, Y = np.meshgrid(x, x)
         mask = X * X + Y * Y <= (0.5 * info.acquisition_fov) ** 2
         for prec, vol in vols.items():
-            n = ref.shape[0]
-            d = (vol[s0:s0 + n].cpu().numpy().astype(np.float64) - ref) / (S.MU_WATER / 1000.0)
-            d = d[:, mask]
+            ds = []
+            for s0, ref in refs:
+                n = ref.shape[0]
+                d = (vol[s0:s0 + n].cpu().numpy().astype(np.float64) - ref) / (S.MU_WATER / 1000.0)
+                ds.append(d[:, mask].ravel())
+            d = np.concatenate(ds)
             hu[prec] = {"rmse": float(np.sqrt(np.mean(d * d))), "max": float(np.abs(d).max()),
-                        "slices": [int(s0), int(s0 + n - 1)]}
+                        "slices": [[int(s0), int(s0 + r.shape[0] - 1)] for s0, r in refs]}
         log("hu", hu)
 
     bpv = BYTES_PER_VV[args.precision]
-    achieved = n_vv * bpv / (chosen["bp_ms"] * 1e-3)
-    # HBM bytes per launch of the timed kernel, from the calibrated
-    # FETCH_SIZE / WRITE_SIZE passes committed under profiles/.
-    traffic = None
-    tp = os.path.join(ROOT, "profiles", "backproject_full_traffic.json")
-    tkey = args.precision if args.precision == "fp32" else args.precision + "_plan"
-    if os.path.exists(tp):
-        try:
-            traffic = json.load(open(tp)).get(tkey, {}).get("traffic_bytes")
-        except Exception:
-            traffic = None
+    gather = n_vv * bpv / (chosen["bp_ms"] * 1e-3)
+    counters = None
+    if not args.no_counters:
+        del plans, p0, vols
+        torch.cuda.empty_cache()
+        counters = measure_counters(args, chosen.get("tile_z", info.tile_z))
+    key = "bp_" + args.precision
+    roof = {"bound": "hbm", "kernel": f"bp_v1_kernel<{chosen.get('tile_z', info.tile_z)}, "
+                                      f"{args.precision.upper()}>",
+            "achieved": None, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": None,
+            "traffic": None, "bp_ms": chosen["bp_ms"],
+            "gather_rate": {"GBps": gather / 1e9, "bytes_per_voxel_view": bpv,
+                            "note": "4 bilinear taps per voxel-view read from L1/L2 (the "
+                                    "algorithmic gather rate, not HBM traffic)"},
+            "lds_bytes": 0}
+    if counters and key in counters:
+        c = counters[key]
+        hbm = c["fetch_bytes"] + c["write_bytes"]
+        roof.update({"achieved": hbm / (chosen["bp_ms"] * 1e-3) / 1e9,
+                     "frac": hbm / (chosen["bp_ms"] * 1e-3) / HBM_PEAK, "traffic": hbm,
+                     "fetch_bytes": c["fetch_bytes"], "write_bytes": c["write_bytes"],
+                     "valu_busy": c.get("valu_busy"),
+                     "valu_insts_per_voxel_view_wave": c.get("valu_insts", 0) * 64.0 / n_vv,
+                     "limiter": "VALU issue (valu_busy); HBM at frac of peak",
+                     "counters": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE / SQ_ACTIVE_INST_VALU "
+                                 "on a child run of this build (bench.py --probe), calibrated "
+                                 "by the abs-max pass (reads the raw scan) and the thickening "
+                                 "(writes the volume)"})
     out = {
         "metric": "voxel-views/s + s/volume at 512^2, mixed+batched vs plain FP32, <=1 HU RMSE",
         "value": chosen["voxel_views_per_s"],
@@ -528,23 +642,138 @@ def main():
         "config": {"workload": "full: 888ch x 64 rows, 9840 views (984/rot, pitch 0.984), "
                                "512x512x320 slices at 0.625 mm, FOV 500 mm, ramp",
                    "voxel_views": n_vv, "voxel_views_nominal": nominal_voxel_views(w, info),
-                   "path": args.path,
+                   "path": ("batch of one job: the whole chain recorded in a hipGraph and "
+                            "replayed, tile depth from the autotuner (the persistent multi-job "
+                            "kernel is measured by --workload batch)")
+                           if args.path == "batch" else "single plan, stage calls",
                    "tile_z": chosen.get("tile_z", info.tile_z), "autotune_ms": tuned},
         "s_per_volume": chosen["ms_per_step"] * 1e-3,
+        "s_per_volume_e2e": e2e[args.precision],
+        "e2e_note": "scan in pinned host memory, chunked H2D overlapped with rebin+filter and "
+                    "backprojection, volume copied back to pinned host memory",
         "plain_fp32": {"ms_per_step": res["fp32"]["ms_per_step"],
                        "s_per_volume": res["fp32"]["ms_per_step"] * 1e-3,
+                       "s_per_volume_e2e": e2e["fp32"],
                        "voxel_views_per_s": res["fp32"]["voxel_views_per_s"],
                        "bp_ms": res["fp32"]["bp_ms"]},
         "speedup_vs_plain_fp32": res["fp32"]["ms_per_step"] / chosen["ms_per_step"],
         "paths": res,
-        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": traffic, "bp_ms": chosen["bp_ms"],
-                     "bytes_per_voxel_view": bpv},
+        "roofline": roof,
+        "counters": counters,
         "cpu_baseline": cpu,
         "hu_error_vs_fp64": hu,
     }
     print(json.dumps(out), flush=True)
+
+
+def traffic_probe(args, torch):
+    """Child of bench.py's counter runs (rocprofv3 --pmc): one FP16 and one
+    FP32 pass of rebin+filter -> backprojection -> thickening over the full
+    workload, nothing else on the GPU but the scan generation and the plans'
+    set-up kernels."""
+    from freect_amd import scans as S
+    from freect_amd import wfbp
+    w = S.full()
+    ctx = wfbp.Context(0)
+    raw = ctx.phantom_project(w.scan, w.phantom, w.i0, w.seed)
+    torch.cuda.synchronize()
+    for prec in ("fp16", "fp32"):
+        plan = wfbp.Plan(ctx, w.scan, w.recon, precision=prec, tile_z=args.tile_z)
+        filt, scale = plan.alloc_filtered()
+        nat, vol = plan.alloc_native(), plan.alloc_volume()
+        plan.rebin_filter(raw, filt, scale)
+        plan.backproject(filt, scale, nat)
+        plan.thicken(nat, vol)
+        plan.synchronize()
+        info = plan.info
+        del filt, scale, nat, vol, plan
+        torch.cuda.empty_cache()
+    print(json.dumps({"raw_bytes": info.raw_elems * 4, "volume_bytes": info.volume_elems * 4}),
+          flush=True)
+
+
+def _read_counter_csv(d):
+    """{(dispatch, kernel): {counter: value}} from a rocprofv3 csv output tree."""
+    import csv
+    import glob
+    rows = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                k = (int(r.get("Dispatch_Id") or 0), r.get("Kernel_Name", ""))
+                rows.setdefault(k, {})
+                name = r.get("Counter_Name", "")
+                rows[k][name] = rows[k].get(name, 0.0) + float(r.get("Counter_Value") or 0.0)
+    return rows
+
+
+def measure_counters(args, tile_z, simds=1024):
+    """HBM bytes and VALU activity per backprojection dispatch of this
+    build, from three rocprofv3 --pmc passes over a child run (--probe):
+    FETCH_SIZE, WRITE_SIZE, and SQ_INSTS_VALU + SQ_ACTIVE_INST_VALU +
+    GRBM_GUI_ACTIVE.  Calibrated against dispatches of known traffic: the
+    abs-max pass reads the raw scan once, the thickening writes the volume
+    once.  valu_busy = 4 SQ_ACTIVE_INST_VALU / (SIMDs x GRBM_GUI_ACTIVE), the
+    fraction of the dispatch's cycles in which the vector ALUs issue.  None if
+    rocprofv3 is unavailable or a pass fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    rp = shutil.which("rocprofv3")
+    if rp is None:
+        log("counters: rocprofv3 not found")
+        return None
+    passes = [["FETCH_SIZE"], ["WRITE_SIZE"],
+              ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "GRBM_GUI_ACTIVE"]]
+    merged = {}
+    meta = None
+    for counters in passes:
+        d = tempfile.mkdtemp(prefix="fct_pmc_")
+        cmd = ["timeout", "-s", "KILL", "180", rp, "--pmc", *counters, "--output-format", "csv",
+               "-d", d, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__),
+               "--probe", "--tile-z", str(tile_z)]
+        log("counters:", " ".join(counters))
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        for line in r.stdout.splitlines():
+            if line.startswith("{"):
+                meta = json.loads(line)
+        rows = _read_counter_csv(d)
+        shutil.rmtree(d, ignore_errors=True)
+        if not rows or meta is None:
+            log("counters: pass", counters, "failed (rc", r.returncode, ")", r.stderr[-400:])
+            return None
+        for k, v in rows.items():
+            merged.setdefault(k, {}).update(v)
+    order = sorted(merged)
+
+    def first(pred):
+        for k in order:
+            if pred(k[1]):
+                return merged[k]
+        return None
+
+    absmax = first(lambda n: "absmax_kernel" in n)
+    thk = first(lambda n: "thicken_kernel" in n)
+    bps = [merged[k] for k in order if "bp_v1_kernel" in k[1]]
+    if absmax is None or thk is None or len(bps) != 2:
+        log("counters: unexpected dispatches", [k[1][:60] for k in order])
+        return None
+    fscale = meta["raw_bytes"] / absmax["FETCH_SIZE"]
+    wscale = meta["volume_bytes"] / thk["WRITE_SIZE"]
+    out = {"calibration": {"fetch_bytes_per_unit": fscale, "write_bytes_per_unit": wscale}}
+    for name, c in zip(("bp_fp16", "bp_fp32"), bps):
+        out[name] = {"fetch_bytes": c["FETCH_SIZE"] * fscale,
+                     "write_bytes": c["WRITE_SIZE"] * wscale,
+                     "valu_insts": c.get("SQ_INSTS_VALU"),
+                     "valu_busy": 4.0 * c["SQ_ACTIVE_INST_VALU"] / (simds * c["GRBM_GUI_ACTIVE"])
+                     if c.get("GRBM_GUI_ACTIVE") else None}
+    rf = [merged[k] for k in order if "rebin_filter" in k[1]]
+    if len(rf) == 2:
+        for name, c in zip(("rebin_filter_fp16", "rebin_filter_fp32"), rf):
+            out[name] = {"fetch_bytes": c["FETCH_SIZE"] * fscale,
+                         "write_bytes": c["WRITE_SIZE"] * wscale}
+    log("counters", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
     tile = (a.sel_iy / 8) * a.tiles_x + a.sel_ix / 8;
     slab = a.sel_is / TZ;
   } else {
-    const int lin = xcd_linear_block(blockIdx.x, gridDim.x);
+    const int lin = a.slab0 * a.tiles_x * a.tiles_y + xcd_linear_block(blockIdx.x, gridDim.x);
     const int tiles = a.tiles_x * a.tiles_y;
     slab = lin / tiles;
     tile = lin - slab * tiles;
@@ -205,7 +205,8 @@ __global__ __launch_bounds__(64) void bp_kernel(BpArgs a) {
 
 template <int TZ, int PREC>
 hipError_t launch_tz_prec(const BpArgs& a, int mode, hipStream_t s) {
-  const int blocks = mode == kBpSelect ? 1 : a.tiles_x * a.tiles_y * a.nslabs;
+  const int blocks =
+      mode == kBpSelect ? 1 : a.tiles_x * a.tiles_y * (a.slab_count ? a.slab_count : a.nslabs);
   if (mode == kBpAccumulate)
     hipLaunchKernelGGL((bp_kernel<TZ, PREC, kBpAccumulate>), dim3(blocks), dim3(64), 0, s, a);
   else if (mode == kBpCount)

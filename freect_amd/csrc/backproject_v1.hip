@@ -616,7 +616,8 @@ template <int TZ, int PREC, int MODE, bool CENTERED>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(TZ == 8 ? bp_min_waves<TZ, PREC>() : 1, 8)))
 void bp_v1_kernel(BpArgs a) {
-  bp_v1_tile<TZ, PREC, MODE, CENTERED>(a, xcd_block(blockIdx.x, gridDim.x));
+  bp_v1_tile<TZ, PREC, MODE, CENTERED>(
+      a, a.slab0 * a.tiles_x * a.tiles_y + xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Persistent backprojection of many jobs in one launch.  The tiles of all
@@ -655,7 +656,7 @@ void bp_batch_kernel(BpBatchArgs b,
 
 template <int TZ, int PREC, bool C>
 hipError_t launch3(const BpArgs& a, int mode, hipStream_t s) {
-  const int blocks = a.tiles_x * a.tiles_y * a.nslabs;
+  const int blocks = a.tiles_x * a.tiles_y * (a.slab_count ? a.slab_count : a.nslabs);
   if (mode == kBpAccumulate)
     hipLaunchKernelGGL((bp_v1_kernel<TZ, PREC, kBpAccumulate, C>), dim3(blocks), dim3(64), 0, s, a);
   else if (mode == kBpCount)

@@ -156,6 +156,12 @@ void free_plan_buffers(fct_wfbp_gpu_plan* p) {
                   p->d_thkT_w, p->d_wrange, p->d_zbounds};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  for (int b = 0; b < 2; ++b) {
+    if (p->h_stage[b]) (void)hipHostFree(p->h_stage[b]);
+    if (p->ev_stage[b]) (void)hipEventDestroy(p->ev_stage[b]);
+  }
+  if (p->ev_compute) (void)hipEventDestroy(p->ev_compute);
+  if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
 }
 
 // Views a slab of tz slices can select: table z within one feed of the
@@ -204,6 +210,7 @@ int plan_set_tile_z(fct_wfbp_gpu_plan* p, int tz) {
   HIP_TRY(p->ctx, upload(&d, vr, p->stream));
   if (p->d_slab_vrange) (void)hipFree(p->d_slab_vrange);
   p->d_slab_vrange = d;
+  p->h_slab_vrange = vr;
   p->tile_z = tz;
   p->nslabs = (int)vr.size();
   return FCT_OK;
@@ -238,6 +245,211 @@ BpArgs plan_bp_args(const fct_wfbp_gpu_plan* p) {
 namespace {
 
 fct::BpArgs bp_args(const fct_wfbp_gpu_plan* p) { return fct::plan_bp_args(p); }
+
+// Rebin + filter of every line of the scan (line0 / nlines narrow it).
+fct::RebinFilterArgs rf_args(const fct_wfbp_gpu_plan* p, const float* raw, void* out,
+                             float* scale) {
+  fct::RebinFilterArgs a{};
+  a.raw = raw;
+  a.out = out;
+  a.scale = scale;
+  a.hsum = (float)p->hsum;
+  a.raw_elems = p->raw_elems();
+  a.tab = p->d_rebin;
+  a.Hn = p->d_Hn;
+  a.tw = p->d_tw;
+  a.nviews = p->nviews;
+  a.nrows = p->nrows;
+  a.ncols = p->ncols;
+  a.cpitch = p->cpitch;
+  a.nfft = p->nfft;
+  a.log2n = p->log2n;
+  a.precision = p->opts.precision;
+  a.nlines = p->nviews * p->nrows;
+  a.ncu = p->ncu;
+  a.raw_nrows = p->raw_scan.n_rows;
+  a.raw_ncols = p->raw_scan.n_channels;
+  a.states = p->ffs.S;
+  a.zshift = fct_ffs::has_z(p->ffs.mode) ? 1 : 0;
+  return a;
+}
+
+// ---- Streaming whole chain ------------------------------------------------
+// The reference copies every frame into one host array
+// (ReconstructionMachine.cu:88-93), flips it on one thread (:150-162) and
+// copies each row sheet synchronously before rebinning it (:246).  Here the
+// views arrive in chunks: each chunk is copied to the device on the plan's
+// copy stream (from the caller's array, or from pinned staging a view
+// reader fills while the previous chunk is in flight), and on the plan's
+// stream the parallel views whose raw views have all arrived are rebinned
+// and filtered, and groups of slabs whose views are all filtered are
+// backprojected, while later chunks are still being copied.  Each line pair
+// and each tile is computed by the same code as in the one-shot path.
+// FP16 / BF16 storage needs a bound on max|raw| before the first line is
+// stored: a caller's bound, or the running abs-max of the views so far, in
+// which case the quads already stored are rescaled (by a power of two) when
+// a later chunk raises the storage scale.
+struct ViewSource {
+  const float* host;       // caller's scan, [view][row][ch]
+  fct_view_reader read;    // or a reader filling pinned staging
+  void* user;
+};
+
+constexpr int kStreamViews = 256;       // default views per chunk
+constexpr int kStreamMinSlabs = 4;      // smallest backprojection group
+
+int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
+                          float max_abs_raw, float* volume_host) {
+  fct_wfbp_gpu_ctx* ctx = p->ctx;
+  if (vpc <= 0) vpc = kStreamViews;
+  const int nv = p->raw_scan.n_views;
+  vpc = std::min(vpc, nv);
+  const size_t per_view = (size_t)p->raw_scan.n_rows * p->raw_scan.n_channels;
+  const size_t plane = (size_t)p->recon.ny * p->recon.nx;
+  const size_t nat_bytes = (size_t)p->n_native * plane * sizeof(float);
+  const size_t vol_bytes = (size_t)p->n_req * plane * sizeof(float);
+  if (!p->d_filtered) HIP_TRY(ctx, hipMalloc(&p->d_filtered, p->filtered_bytes()));
+  if (p->scale_elems() && !p->d_scale)
+    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&p->d_scale), p->scale_elems() * sizeof(float)));
+  if (!p->d_native) HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&p->d_native), nat_bytes));
+  if (!p->d_raw_stage)
+    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&p->d_raw_stage),
+                           (size_t)p->raw_elems() * sizeof(float)));
+  if (!p->d_vol_stage) HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&p->d_vol_stage), vol_bytes));
+  if (!p->copy_stream) HIP_TRY(ctx, hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking));
+  for (int b = 0; b < 2; ++b)
+    if (!p->ev_stage[b]) HIP_TRY(ctx, hipEventCreateWithFlags(&p->ev_stage[b], hipEventDisableTiming));
+  if (!p->ev_compute) HIP_TRY(ctx, hipEventCreateWithFlags(&p->ev_compute, hipEventDisableTiming));
+  if (src.read && p->stage_views != vpc) {
+    for (int b = 0; b < 2; ++b) {
+      if (p->h_stage[b]) HIP_TRY(ctx, hipHostFree(p->h_stage[b]));
+      p->h_stage[b] = nullptr;
+    }
+    p->stage_views = 0;
+    for (int b = 0; b < 2; ++b)
+      HIP_TRY(ctx, hipHostMalloc(reinterpret_cast<void**>(&p->h_stage[b]),
+                                 (size_t)vpc * per_view * sizeof(float), hipHostMallocDefault));
+    p->stage_views = vpc;
+  }
+  // The copies overwrite the raw buffer only after the plan's earlier work
+  // (which may still read it) has finished.
+  HIP_TRY(ctx, hipEventRecord(p->ev_compute, p->stream));
+  HIP_TRY(ctx, hipStreamWaitEvent(p->copy_stream, p->ev_compute, 0));
+
+  const int prec = p->opts.precision;
+  const bool q16 = prec != FCT_PRECISION_FP32;
+  const bool grow = q16 && !(max_abs_raw > 0.0f);
+  if (q16) {
+    HIP_TRY(ctx, fct::launch_fill(p->d_scale, fct::kScaleElems, 0.0f, p->stream));
+    if (!grow) {
+      HIP_TRY(ctx, fct::launch_scale_set(p->d_scale, max_abs_raw, p->stream));
+      HIP_TRY(ctx, fct::launch_scale_update(p->d_scale, (float)p->hsum, p->stream));
+    }
+  }
+  fct::RebinFilterArgs rf = rf_args(p, p->d_raw_stage, p->d_filtered, p->d_scale);
+  rf.skip_scale = 1;
+  fct::BpArgs bp = bp_args(p);
+  bp.filt = p->d_filtered;
+  bp.filt_scale = p->d_scale;
+  bp.native = p->d_native;
+  const int S = p->ffs.S, nrows = p->nrows, npar = p->nviews;
+  const size_t entries_per_view = (size_t)nrows * p->cpitch;
+  int j_done = 0, s_done = 0;
+
+  auto rebin_upto = [&](int j1) -> hipError_t {
+    if (j1 <= j_done) return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (grow) {
+      e = fct::launch_scale_update(p->d_scale, (float)p->hsum, p->stream);
+      if (e == hipSuccess)
+        e = fct::launch_quad_rescale(p->d_filtered, (size_t)j_done * entries_per_view, prec,
+                                     p->d_scale, p->ncu, p->stream);
+      if (e != hipSuccess) return e;
+    }
+    rf.line0 = j_done * nrows;
+    rf.nlines = (j1 - j_done) * nrows;
+    e = fct::launch_rebin_filter(rf, p->stream);
+    j_done = j1;
+    return e;
+  };
+  auto bp_ready = [&](bool final) -> hipError_t {
+    int s1 = s_done;
+    if (final) {
+      s1 = p->nslabs;
+    } else {
+      while (s1 < p->nslabs) {
+        const int2 v = p->h_slab_vrange[s1];
+        if (v.y > j_done && v.x < v.y) break;
+        ++s1;
+      }
+      if (s1 - s_done < kStreamMinSlabs) return hipSuccess;
+    }
+    if (s1 <= s_done) return hipSuccess;
+    bp.slab0 = s_done;
+    bp.slab_count = s1 - s_done;
+    s_done = s1;
+    return fct::launch_backproject(bp, p->tile_z, p->variant, prec, fct::kBpAccumulate, p->stream);
+  };
+  auto drain = [&]() {
+    (void)hipStreamSynchronize(p->copy_stream);
+    (void)hipStreamSynchronize(p->stream);
+  };
+
+  for (int v0 = 0, k = 0; v0 < nv; v0 += vpc, ++k) {
+    const int n = std::min(vpc, nv - v0), b = k & 1;
+    float* dst = p->d_raw_stage + (size_t)v0 * per_view;
+    const size_t bytes = (size_t)n * per_view * sizeof(float);
+    hipError_t e = hipSuccess;
+    if (src.read) {
+      if (k >= 2) e = hipEventSynchronize(p->ev_stage[b]);   // staging b free again
+      if (e == hipSuccess && src.read(src.user, v0, n, p->h_stage[b]) != 0) {
+        drain();
+        set_err(ctx, "the view reader failed for views [%d, %d)", v0, v0 + n);
+        return FCT_ERR_INVALID_ARGUMENT;
+      }
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(dst, p->h_stage[b], bytes, hipMemcpyHostToDevice, p->copy_stream);
+    } else {
+      e = hipMemcpyAsync(dst, src.host + (size_t)v0 * per_view, bytes, hipMemcpyHostToDevice,
+                         p->copy_stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(p->ev_stage[b], p->copy_stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(p->stream, p->ev_stage[b], 0);
+    const int arrived = v0 + n;
+    if (e == hipSuccess && grow)
+      e = fct::launch_absmax(dst, (int64_t)n * (int64_t)per_view, p->d_scale, p->ncu, p->stream);
+    // Parallel views j with j S + reach <= arrived have every raw view they read.
+    int j1 = npar;
+    if (arrived < nv) {
+      const int d = arrived - p->rebin_reach;
+      j1 = d < 0 ? 0 : std::min(npar, d / S + 1);
+      if (nrows & 1) j1 &= ~1;   // line pairs must not straddle a chunk end
+    }
+    if (e == hipSuccess) e = rebin_upto(j1);
+    if (e == hipSuccess) e = bp_ready(false);
+    if (e != hipSuccess) {
+      drain();
+      set_err(ctx, "streaming reconstruction: %s", hipGetErrorString(e));
+      return FCT_ERR_DEVICE;
+    }
+  }
+  hipError_t e = rebin_upto(npar);
+  if (e == hipSuccess) e = bp_ready(true);
+  if (e != hipSuccess) {
+    drain();
+    set_err(ctx, "streaming reconstruction: %s", hipGetErrorString(e));
+    return FCT_ERR_DEVICE;
+  }
+  int st = fct_wfbp_gpu_thicken(p, p->d_native, p->d_vol_stage);
+  if (st) {
+    drain();
+    return st;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(volume_host, p->d_vol_stage, vol_bytes, hipMemcpyDeviceToHost,
+                              p->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(p->stream));
+  return FCT_OK;
+}
 
 }  // namespace
 
@@ -428,6 +640,9 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
       tab[(size_t)zb * p->ncols + c] =
           fct::RebinDev{e.b0, e.b1, e.aoff, e.valid ? e.rho + 1 : 0, (float)e.fb, (float)e.fa,
                         e.mmax, 0};
+      // Parallel view j reads raw projections up to (j + aoff + 1) S + rho.
+      if (e.valid)
+        p->rebin_reach = std::max(p->rebin_reach, (e.aoff + 1) * p->ffs.S + e.rho + 1);
     }
 
   // Filter spectrum (generate_filter, ReconstructionMachine.cu:478-544):
@@ -500,6 +715,7 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   PLAN_TRY(upload(&p->d_zs, zs, p->stream));
   PLAN_TRY(upload(&p->d_zt, zt, p->stream));
   PLAN_TRY(upload(&p->d_slab_vrange, vr, p->stream));
+  p->h_slab_vrange = vr;
   PLAN_TRY(upload(&p->d_wrange, wrange, p->stream));
   PLAN_TRY(upload(&p->d_rebin, tab, p->stream));
   PLAN_TRY(upload(&p->d_Hn, Hn, p->stream));
@@ -573,29 +789,7 @@ FCT_API int fct_wfbp_gpu_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, vo
     return set_err(p->ctx, "the scale buffer is required for FP16/BF16"),
            FCT_ERR_INVALID_ARGUMENT;
   HIP_TRY(p->ctx, hipSetDevice(p->device));
-  fct::RebinFilterArgs a{};
-  a.raw = raw;
-  a.out = filtered;
-  a.scale = scale;
-  a.hsum = (float)p->hsum;
-  a.raw_elems = p->raw_elems();
-  a.tab = p->d_rebin;
-  a.Hn = p->d_Hn;
-  a.tw = p->d_tw;
-  a.nviews = p->nviews;
-  a.nrows = p->nrows;
-  a.ncols = p->ncols;
-  a.cpitch = p->cpitch;
-  a.nfft = p->nfft;
-  a.log2n = p->log2n;
-  a.precision = p->opts.precision;
-  a.nlines = p->nviews * p->nrows;
-  a.ncu = p->ncu;
-  a.raw_nrows = p->raw_scan.n_rows;
-  a.raw_ncols = p->raw_scan.n_channels;
-  a.states = p->ffs.S;
-  a.zshift = fct_ffs::has_z(p->ffs.mode) ? 1 : 0;
-  HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
+  HIP_TRY(p->ctx, fct::launch_rebin_filter(rf_args(p, raw, filtered, scale), p->stream));
   return FCT_OK;
 }
 
@@ -776,34 +970,32 @@ FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* p, const float* raw, flo
                                      int32_t host_buffers) {
   if (!p || !raw || !volume) return FCT_ERR_INVALID_ARGUMENT;
   HIP_TRY(p->ctx, hipSetDevice(p->device));
-  const size_t raw_bytes = (size_t)p->raw_elems() * sizeof(float);
+  if (host_buffers) return reconstruct_streaming(p, ViewSource{raw, nullptr, nullptr}, 0, 0.0f,
+                                                 volume);
   const size_t nat_bytes = (size_t)p->n_native * p->recon.ny * p->recon.nx * sizeof(float);
-  const size_t vol_bytes = (size_t)p->n_req * p->recon.ny * p->recon.nx * sizeof(float);
   if (!p->d_filtered) HIP_TRY(p->ctx, hipMalloc(&p->d_filtered, p->filtered_bytes()));
   if (p->scale_elems() && !p->d_scale)
     HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_scale),
                               p->scale_elems() * sizeof(float)));
   if (!p->d_native) HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_native), nat_bytes));
-  const float* d_raw = raw;
-  float* d_vol = volume;
-  if (host_buffers) {
-    if (!p->d_raw_stage) HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_raw_stage), raw_bytes));
-    if (!p->d_vol_stage) HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_vol_stage), vol_bytes));
-    HIP_TRY(p->ctx, hipMemcpyAsync(p->d_raw_stage, raw, raw_bytes, hipMemcpyHostToDevice, p->stream));
-    d_raw = p->d_raw_stage;
-    d_vol = p->d_vol_stage;
-  }
-  int st = fct_wfbp_gpu_rebin_filter(p, d_raw, p->d_filtered, p->d_scale);
+  int st = fct_wfbp_gpu_rebin_filter(p, raw, p->d_filtered, p->d_scale);
   if (st) return st;
   st = fct_wfbp_gpu_backproject(p, p->d_filtered, p->d_scale, p->d_native);
   if (st) return st;
-  st = fct_wfbp_gpu_thicken(p, p->d_native, d_vol);
-  if (st) return st;
-  if (host_buffers) {
-    HIP_TRY(p->ctx, hipMemcpyAsync(volume, d_vol, vol_bytes, hipMemcpyDeviceToHost, p->stream));
-    HIP_TRY(p->ctx, hipStreamSynchronize(p->stream));
-  }
-  return FCT_OK;
+  return fct_wfbp_gpu_thicken(p, p->d_native, volume);
+}
+
+FCT_API int fct_wfbp_gpu_reconstruct_stream(fct_wfbp_gpu_plan* p, fct_view_reader read,
+                                            void* user, int32_t views_per_chunk,
+                                            float max_abs_raw, float* volume_host) {
+  if (!p || !read || !volume_host) return FCT_ERR_INVALID_ARGUMENT;
+  if (views_per_chunk < 0)
+    return set_err(p->ctx, "views_per_chunk must be >= 0"), FCT_ERR_INVALID_ARGUMENT;
+  if (!(max_abs_raw >= 0.0f) || !(max_abs_raw <= 3.0e38f))
+    return set_err(p->ctx, "max_abs_raw must be finite and >= 0"), FCT_ERR_INVALID_ARGUMENT;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  return reconstruct_streaming(p, ViewSource{nullptr, read, user}, views_per_chunk, max_abs_raw,
+                               volume_host);
 }
 
 FCT_API int fct_wfbp_gpu_count_voxel_views(fct_wfbp_gpu_plan* p, int32_t* per_voxel,

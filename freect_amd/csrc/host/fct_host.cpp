@@ -168,17 +168,35 @@ bool RawDataSet::projectionIsAvailable(int i) const {
          m_is_loaded[i];
 }
 
+bool RawDataSet::headerIsAvailable(int i) const {
+  return projectionIsAvailable(i) ||
+         (i >= 0 && i < (int)m_total_num_projections && i < (int)m_has_header.size() &&
+          m_has_header[i]);
+}
+
 float RawDataSet::getTablePosition(int i) const {
-  return projectionIsAvailable(i) ? m_data[i]->m_dfc_axial_position : NAN;
+  return headerIsAvailable(i) ? m_data[i]->m_dfc_axial_position : NAN;
 }
 
 float RawDataSet::getTubeAngle(int i) const {
-  return projectionIsAvailable(i) ? m_data[i]->m_dfc_angular_position : NAN;
+  return headerIsAvailable(i) ? m_data[i]->m_dfc_angular_position : NAN;
 }
 
 void RawDataSet::copyProjection(int i, float* buffer) const {
   std::memcpy(buffer, m_data[i]->m_projection.data(),
               (size_t)m_detector_rows * m_detector_channels * sizeof(float));
+}
+
+int RawDataSet::readViews(int v0, int n, float* dst) {
+  const size_t per = (size_t)m_detector_rows * m_detector_channels;
+  for (int v = v0; v < v0 + n; ++v) {
+    if (!projectionIsAvailable(v)) {
+      const int st = readProjection(v);
+      if (st) return st;
+    }
+    copyProjection(v, dst + per * (size_t)(v - v0));
+  }
+  return FCT_OK;
 }
 
 void RawDataSet::printMetadata() const {
@@ -291,6 +309,78 @@ int FreeCTDataSet::readMetadata() {
   m_data.clear();
   m_data.resize(m_total_num_projections);
   m_is_loaded.assign(m_total_num_projections, false);
+  m_has_header.assign(m_total_num_projections, false);
+  m_in.reset();
+  m_next_frame = -1;
+  return FCT_OK;
+}
+
+int FreeCTDataSet::readHeaders() {
+  std::ifstream in(m_path + "/projections.dat", std::ios::binary);
+  if (!in) {
+    m_last_error = "cannot open " + m_path + "/projections.dat";
+    return FCT_ERR_INVALID_ARGUMENT;
+  }
+  const size_t frame = 3 * sizeof(float) + (size_t)m_detector_rows * m_detector_channels * sizeof(float);
+  in.seekg(0, std::ios::end);
+  if ((size_t)in.tellg() < frame * m_total_num_projections) {
+    m_last_error = "projections.dat is shorter than total_num_projections frames";
+    return FCT_ERR_INVALID_ARGUMENT;
+  }
+  for (int i = 0; i < (int)m_total_num_projections; ++i) {
+    float hdr[3];
+    in.seekg((std::streamoff)(frame * (size_t)i));
+    in.read(reinterpret_cast<char*>(hdr), sizeof hdr);
+    if (!in) {
+      m_last_error = "cannot read the header of frame " + std::to_string(i);
+      return FCT_ERR_INVALID_ARGUMENT;
+    }
+    if (!m_data[i]) m_data[i] = std::make_unique<RawDataFrame>();
+    RawDataFrame& f = *m_data[i];
+    f.m_detector_rows = m_detector_rows;
+    f.m_detector_channels = m_detector_channels;
+    f.m_dfc_angular_position = hdr[0];
+    f.m_dfc_axial_position = hdr[1];
+    f.m_dfc_radial_distance = hdr[2];
+    m_has_header[i] = true;
+  }
+  return FCT_OK;
+}
+
+std::ifstream* FreeCTDataSet::stream() {
+  if (!m_in) {
+    m_in = std::make_unique<std::ifstream>(m_path + "/projections.dat", std::ios::binary);
+    m_next_frame = -1;
+  }
+  return *m_in ? m_in.get() : nullptr;
+}
+
+int FreeCTDataSet::readViews(int v0, int n, float* dst) {
+  if (v0 < 0 || n < 0 || v0 + n > (int)m_total_num_projections) return FCT_ERR_INVALID_ARGUMENT;
+  const size_t per = (size_t)m_detector_rows * m_detector_channels;
+  const size_t frame = 3 * sizeof(float) + per * sizeof(float);
+  for (int v = v0; v < v0 + n; ++v) {
+    float* out = dst + per * (size_t)(v - v0);
+    if (projectionIsAvailable(v)) {
+      copyProjection(v, out);
+      continue;
+    }
+    std::ifstream* in = stream();
+    if (!in) {
+      m_last_error = "cannot open " + m_path + "/projections.dat";
+      return FCT_ERR_INVALID_ARGUMENT;
+    }
+    if (m_next_frame != v) in->seekg((std::streamoff)(frame * (size_t)v));
+    float hdr[3];
+    in->read(reinterpret_cast<char*>(hdr), sizeof hdr);
+    in->read(reinterpret_cast<char*>(out), per * sizeof(float));
+    if (!*in) {
+      m_last_error = "projections.dat is shorter than total_num_projections frames";
+      m_in.reset();
+      return FCT_ERR_INVALID_ARGUMENT;
+    }
+    m_next_frame = v + 1;
+  }
   return FCT_OK;
 }
 
@@ -556,7 +646,12 @@ int ReconstructionMachine::LoadRawData() {
     m_org_data_set = std::make_shared<FreeCTDataSet>();
   m_org_data_set->setPath(m_rp.raw_data_dir);
   int st = m_org_data_set->initialize();
-  if (!st) st = m_org_data_set->readAll();
+  if (!st) {
+    if (auto* f = dynamic_cast<FreeCTDataSet*>(m_org_data_set.get()))
+      st = f->readHeaders();   // projections stream from disk during the upload
+    else
+      st = m_org_data_set->readAll();
+  }
   if (st) m_err = m_org_data_set->m_last_error;
   return st;
 }
@@ -565,10 +660,16 @@ int ReconstructionMachine::LoadRawData() {
 // tables (ReconstructionMachine.cu:96-102) that its backprojection indexes
 // per view (cuda_kernels.cuh:230,235); its rebinning assumes uniform views
 // (cuda_kernels.cuh:15-19).  The plan takes the law
-// a_v = angle0 + rotation*v*2pi/ppr, z_v = z0 + v*feed/ppr: this derives it
-// from views 0, 1 and n-1 and checks it against EVERY view, within 2 % of a
-// view step in angle and 2 % of the collimated slice width in z (plus two
-// float32 ulps of the stored value).  A scan off that law (jittered or
+// a_v = angle0 + rotation*v*2pi/ppr, z_v = z0 + v*feed/ppr.  This derives it
+// from the frame headers and checks it against EVERY view: the direction and
+// step from views 0 and S (S focal states; with a flying focal spot the
+// headers carry each state's deflection, RawDataSet.cpp:99-101, a constant
+// offset per state), the feed from the last view in view 0's state, then
+// per focal state the mean residual, and every view's residual from its
+// state's mean must stay within 2 % of a view step in angle and 2 % of the
+// collimated slice width in z (plus two float32 ulps of the stored values).
+// angle0 and z0 are the first view's values plus the mean of the states'
+// offsets (the undeflected source).  A scan off that law (jittered or
 // non-uniform angles or table positions, a step that is not +-2pi/ppr) is
 // rejected with FCT_ERR_UNSUPPORTED rather than reconstructed wrongly.
 int ReconstructionMachine::ViewLaw(const RawDataSet& ds, fct_scan_geometry* g, std::string* err) {
@@ -579,38 +680,63 @@ int ReconstructionMachine::ViewLaw(const RawDataSet& ds, fct_scan_geometry* g, s
     return FCT_ERR_INVALID_ARGUMENT;
   }
   for (int v = 0; v < n; ++v)
-    if (!ds.projectionIsAvailable(v)) {
+    if (!ds.headerIsAvailable(v)) {
       *err = "projection " + std::to_string(v) + " is not loaded";
       return FCT_ERR_INVALID_ARGUMENT;
     }
+  const int S = g->ffs_mode == FCT_FFS_PHI_Z ? 4 : g->ffs_mode ? 2 : 1;
   const double dth = kTwoPi / ppr;
-  g->angle0 = ds.getTubeAngle(0);
-  g->z0 = ds.getTablePosition(0);
+  const double a0 = ds.getTubeAngle(0), z0 = ds.getTablePosition(0);
   g->rotation = 1;
-  if (n > 1) {
-    const double step = std::remainder((double)ds.getTubeAngle(1) - g->angle0, kTwoPi);
+  g->feed = 0.0;
+  const int ref = n > S ? S : (n > 1 ? 1 : 0);
+  if (ref > 0) {
+    const double step = std::remainder((double)ds.getTubeAngle(ref) - a0, kTwoPi) / ref;
     if (std::fabs(step - dth) <= 0.25 * dth) {
       g->rotation = 1;
     } else if (std::fabs(step + dth) <= 0.25 * dth) {
       g->rotation = -1;
     } else {
       char b[160];
-      std::snprintf(b, sizeof b, "tube angle step %.6g rad between views 0 and 1 is not "
-                    "+-2pi/%d", step, ppr);
+      std::snprintf(b, sizeof b, "tube angle step %.6g rad between views 0 and %d is not "
+                    "+-2pi/%d", step, ref, ppr);
       *err = b;
       return FCT_ERR_UNSUPPORTED;
     }
-    g->feed = ((double)ds.getTablePosition(n - 1) - g->z0) * ppr / (n - 1);
+    const int last = (n - 1) / S * S;   // last view in view 0's focal state
+    if (last > 0) g->feed = ((double)ds.getTablePosition(last) - z0) * ppr / last;
   }
+  // Residuals from the law through view 0, their mean per focal state, and
+  // every view's deviation from its state's mean.
+  std::vector<double> ra(n), rz(n), ma(S, 0.0), mz(S, 0.0);
+  std::vector<int> cnt(S, 0);
+  for (int v = 0; v < n; ++v) {
+    ra[v] = std::remainder((double)ds.getTubeAngle(v) - (a0 + g->rotation * v * dth), kTwoPi);
+    rz[v] = (double)ds.getTablePosition(v) - (z0 + v * g->feed / ppr);
+    ma[v % S] += ra[v];
+    mz[v % S] += rz[v];
+    ++cnt[v % S];
+  }
+  double oa = 0.0, oz = 0.0;
+  int ns = 0;
+  for (int k = 0; k < S; ++k)
+    if (cnt[k]) {
+      ma[k] /= cnt[k];
+      mz[k] /= cnt[k];
+      oa += ma[k];
+      oz += mz[k];
+      ++ns;
+    }
+  g->angle0 = a0 + oa / ns;
+  g->z0 = z0 + oz / ns;
   const double csw = g->r * g->row_spacing / g->dsd;
   auto ulp2 = [](float x) { return 2.0 * (double)std::nextafter(std::fabs(x), INFINITY) -
                                    2.0 * (double)std::fabs(x); };
   for (int v = 0; v < n; ++v) {
     const float av = ds.getTubeAngle(v), zv = ds.getTablePosition(v);
-    const double da = std::remainder((double)av - (g->angle0 + g->rotation * v * dth), kTwoPi);
-    const double dz = (double)zv - (g->z0 + v * g->feed / ppr);
-    const double ta = 0.02 * dth + ulp2(av) + ulp2((float)g->angle0);
-    const double tz = 0.02 * csw + ulp2(zv) + ulp2((float)g->z0);
+    const double da = ra[v] - ma[v % S], dz = rz[v] - mz[v % S];
+    const double ta = 0.02 * dth + ulp2(av) + ulp2((float)a0);
+    const double tz = 0.02 * csw + ulp2(zv) + ulp2((float)z0);
     if (!(std::fabs(da) <= ta) || !(std::fabs(dz) <= tz)) {
       char b[256];
       std::snprintf(b, sizeof b, "view %d is off the uniform helical law: angle off by %.3g rad "
@@ -722,17 +848,10 @@ int ReconstructionMachine::RunReconstruction() {
     m_err = "unknown precision '" + m_rp.precision + "'";
     return FCT_ERR_INVALID_ARGUMENT;
   }
-  // AllocateKeyArrays (ReconstructionMachine.cu:84-105): one [view][row][ch] array.
-  const size_t per_view = (size_t)m_geom.n_rows * m_geom.n_channels;
-  std::vector<float> raw(per_view * m_geom.n_views);
-  for (int v = 0; v < m_geom.n_views; ++v) {
-    if (!m_org_data_set->projectionIsAvailable(v)) {
-      m_err = "projection " + std::to_string(v) + " is not loaded";
-      return FCT_ERR_INVALID_ARGUMENT;
-    }
-    m_org_data_set->copyProjection(v, raw.data() + per_view * v);
-  }
-  m_org_data_set.reset();   // ReconstructionMachine.cu:105
+  // AllocateKeyArrays copies every frame into one [view][row][ch] host array
+  // (ReconstructionMachine.cu:84-105); here the data set's streaming
+  // iterator fills the plan's pinned staging chunk by chunk while earlier
+  // chunks are copied to and processed on the GPU.
   fct_wfbp_gpu_ctx* ctx = nullptr;
   int st = fct_wfbp_gpu_create(m_device, &ctx);
   if (st) {
@@ -746,13 +865,34 @@ int ReconstructionMachine::RunReconstruction() {
   if (!st) st = fct_wfbp_gpu_plan_info(plan, &m_info);
   if (!st) {
     m_reconstructed_data.assign((size_t)m_info.volume_elems, 0.0f);
+    RawDataSet* ds = m_org_data_set.get();
+    std::string reader_err;
+    struct Reader {
+      RawDataSet* ds;
+      std::string* err;
+      static int call(void* u, int32_t v0, int32_t n, float* dst) {
+        auto* r = static_cast<Reader*>(u);
+        const int st = r->ds->readViews(v0, n, dst);
+        if (st) *r->err = r->ds->m_last_error;
+        return st;
+      }
+    } reader{ds, &reader_err};
     const auto t0 = std::chrono::steady_clock::now();
-    st = fct_wfbp_gpu_reconstruct(plan, raw.data(), m_reconstructed_data.data(), 1);
+    st = fct_wfbp_gpu_reconstruct_stream(plan, &Reader::call, &reader, 0, 0.0f,
+                                         m_reconstructed_data.data());
     m_seconds_gpu = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (m_verbose && !st)
       std::cout << "Reconstruction (incl. host<->device copies): " << m_seconds_gpu << " s"
                 << std::endl;
+    if (st && !reader_err.empty()) {
+      m_err = reader_err;
+      fct_wfbp_gpu_plan_destroy(plan);
+      fct_wfbp_gpu_destroy(ctx);
+      m_org_data_set.reset();
+      return st;
+    }
   }
+  m_org_data_set.reset();   // ReconstructionMachine.cu:105
   if (st) m_err = fct_wfbp_gpu_last_error(ctx);
   fct_wfbp_gpu_plan_destroy(plan);
   fct_wfbp_gpu_destroy(ctx);

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <string>
@@ -100,12 +101,19 @@ class RawDataSet {
   std::string getFlyingFocalSpotMode() const { return m_flying_focal_spot_mode; }
   uint16_t getProjectionsPerRotation() const { return m_projections_per_rotation; }
   uint16_t getTotalNumProjections() const { return m_total_num_projections; }
-  // NaN when the projection is not available (RawDataSet.h:109-121).
+  // NaN when the projection (or, for a header-only load, its frame header)
+  // is not available (RawDataSet.h:109-121).
   float getTablePosition(int projection_idx) const;
   float getTubeAngle(int projection_idx) const;
   bool projectionIsAvailable(int projection_idx) const;
+  bool headerIsAvailable(int projection_idx) const;
   // rows x channels floats, [row][channel] (RawDataSet.h:136-139).
   void copyProjection(int projection_idx, float* buffer) const;
+  // Streaming iterator: views [v0, v0 + n) as [view][row][channel] float32
+  // into dst, called in view order by the chunked GPU upload in place of
+  // copying every frame into one whole-scan array.  The default copies loaded
+  // frames and reads missing ones with readProjection.
+  virtual int readViews(int v0, int n, float* dst);
 
   // Exact scan law when the data set knows it (synthetic scans); otherwise
   // ReconstructionMachine derives it from the per-view angles and positions.
@@ -115,6 +123,7 @@ class RawDataSet {
   std::string m_path;
   std::vector<std::unique_ptr<RawDataFrame>> m_data;
   std::vector<bool> m_is_loaded;
+  std::vector<bool> m_has_header;   // frame header read (angles, positions)
   std::string m_manufacturer = "SYNTHETIC";
   uint16_t m_detector_rows = 0;
   uint16_t m_detector_channels = 0;
@@ -146,6 +155,15 @@ class FreeCTDataSet : public RawDataSet {
   int readMetadata() override;
   int readProjection(int projection_idx) override;
   int readAll() override;
+  // Only the frame headers (tube angle, table position, radius), so that
+  // the projections stream from disk through readViews during the upload.
+  int readHeaders();
+  int readViews(int v0, int n, float* dst) override;
+
+ private:
+  std::ifstream* stream();
+  std::unique_ptr<std::ifstream> m_in;
+  int64_t m_next_frame = -1;   // frame the open stream is positioned at
 };
 
 // Seeded synthetic helical scan (water cylinder + ellipsoids, analytic line
@@ -189,6 +207,8 @@ class ReconstructionMachine {
   ~ReconstructionMachine();
 
   int LoadReconConfiguration(const std::string& filepath);
+  // FreeCT-format data: frame headers only; the projections are read from
+  // disk chunk by chunk while the reconstruction streams them to the GPU.
   int LoadRawData();
   int ConfigureCTGeometry();
   int RunReconstruction();

@@ -124,10 +124,10 @@ int main(int argc, char** argv) {
     if (rm.ConfigureCTGeometry()) return fail("ConfigureCTGeometry", rm.lastError());
     const fct_scan_geometry& g = rm.geometry();
     std::printf("{\"n_views\": %d, \"n_rows\": %d, \"n_channels\": %d, \"ppr\": %d, "
-                "\"angle0\": %.9g, \"z0\": %.9g, \"feed\": %.9g, \"dsd\": %.9g, \"r\": %.9g, "
-                "\"channel_spacing\": %.9g, \"row_spacing\": %.9g, \"central_channel\": %.9g, "
-                "\"central_row\": %.9g, \"sum\": %.17g, \"angle_last\": %.9g, \"z_last\": %.9g, "
-                "\"rotation\": %d, \"ffs_mode\": %d, \"anode_angle\": %.9g}\n",
+                "\"angle0\": %.17g, \"z0\": %.17g, \"feed\": %.17g, \"dsd\": %.17g, \"r\": %.17g, "
+                "\"channel_spacing\": %.17g, \"row_spacing\": %.17g, \"central_channel\": %.17g, "
+                "\"central_row\": %.17g, \"sum\": %.17g, \"angle_last\": %.9g, \"z_last\": %.9g, "
+                "\"rotation\": %d, \"ffs_mode\": %d, \"anode_angle\": %.17g}\n",
                 g.n_views, g.n_rows, g.n_channels, g.views_per_rotation, g.angle0, g.z0, g.feed,
                 g.dsd, g.r, g.channel_spacing, g.row_spacing, g.central_channel, g.central_row,
                 sum, ds->getTubeAngle(g.n_views - 1), ds->getTablePosition(g.n_views - 1),

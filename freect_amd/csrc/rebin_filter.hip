@@ -98,18 +98,6 @@ __device__ FctCplx* fft_forward(FctCplx* a, FctCplx* b, const FctCplx* __restric
   return a;
 }
 
-// Scale of FP16 / BF16 storage from the abs-max pass: the power of two s with
-// bound / s < 2^15, bound = max|raw| * sum|h| * (1 + 2^-7) (the margin covers
-// the FP32 transform's round-off).  1 for an all-zero (or non-finite) scan.
-__device__ __forceinline__ float storage_scale(const RebinFilterArgs& a) {
-  const float m = __uint_as_float(reinterpret_cast<const uint32_t*>(a.scale)[1]);
-  const float bound = m * a.hsum * 1.0078125f;
-  if (!(bound > 0.0f) || !(bound <= 3.0e38f)) return 1.0f;
-  int e = 0;
-  frexpf(bound, &e);   // bound < 2^e
-  return ldexpf(1.0f, e - 15);
-}
-
 template <class T>
 __device__ __forceinline__ T to_store(float v);
 template <>
@@ -117,6 +105,56 @@ __device__ __forceinline__ __half to_store<__half>(float v) { return __float2hal
 template <>
 __device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(float v) {
   return __float2bfloat16(v);
+}
+
+__device__ __forceinline__ float from_store(__half v) { return __half2float(v); }
+__device__ __forceinline__ float from_store(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+// Scale of FP16 / BF16 storage from the abs-max pass: the power of two s with
+// bound / s < 2^15, bound = max|raw| * sum|h| * (1 + 2^-7) (the margin covers
+// the FP32 transform's round-off).  1 for an all-zero (or non-finite) scan.
+__device__ __forceinline__ float scale_of_bound(const float* scale, float hsum) {
+  const float m = __uint_as_float(reinterpret_cast<const uint32_t*>(scale)[1]);
+  const float bound = m * hsum * 1.0078125f;
+  if (!(bound > 0.0f) || !(bound <= 3.0e38f)) return 1.0f;
+  int e = 0;
+  frexpf(bound, &e);   // bound < 2^e
+  return ldexpf(1.0f, e - 15);
+}
+
+__device__ __forceinline__ float storage_scale(const RebinFilterArgs& a) {
+  return scale_of_bound(a.scale, a.hsum);
+}
+
+// Streaming FP16 / BF16 storage: the bound in scale[1] grows as views
+// arrive.  scale[0] becomes the scale of the bound so far and scale[2] the
+// factor s_old / s_new (a power of two, 1 while the scale is unchanged) by
+// which the quads stored under the previous scale are multiplied.
+__global__ void scale_update_kernel(float* scale, float hsum) {
+  if (threadIdx.x != 0) return;
+  const float s_new = scale_of_bound(scale, hsum);
+  const float s_old = scale[0];
+  scale[2] = s_old > 0.0f ? s_old / s_new : 1.0f;
+  scale[0] = s_new;
+}
+
+// Multiplies n 16-bit values (four per 8-byte entry) by scale[2] when that
+// is not 1: exact for results in the format's normal range.
+template <class T>
+__global__ __launch_bounds__(256) void quad_rescale_kernel(uint2* q, size_t n_entries,
+                                                           const float* scale) {
+  const float r = scale[2];
+  if (r == 1.0f) return;
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n_entries; i += stride) {
+    uint2 v = q[i];
+    T h[4];
+    __builtin_memcpy(h, &v, sizeof v);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = to_store<T>(from_store(h[k]) * r);
+    __builtin_memcpy(&v, h, sizeof v);
+    q[i] = v;
+  }
 }
 
 // Value h of line (view, row r) at channel c into its four quad slots:
@@ -191,6 +229,11 @@ __device__ __forceinline__ uint32_t abs_bits(float v) {
   return u < 0x7f800000u ? u : 0u;
 }
 
+// The FP16 / BF16 bound from a caller's max|raw| (|m| as ordered bits).
+__global__ void scale_set_kernel(float* scale, float m) {
+  if (threadIdx.x == 0) reinterpret_cast<uint32_t*>(scale)[1] = abs_bits(m);
+}
+
 template <bool kVec>
 __global__ __launch_bounds__(256) void absmax_kernel(const float* __restrict__ x, int64_t n,
                                                      float* scale) {
@@ -221,9 +264,9 @@ __global__ __launch_bounds__(kRfThreads) void rebin_filter_kernel(RebinFilterArg
   const int N = a.nfft;
   FctCplx* buf0 = lds;
   FctCplx* buf1 = lds + N;
-  const int l0 = 2 * blockIdx.x;
+  const int l0 = a.line0 + 2 * blockIdx.x;
   const int l1 = l0 + 1;
-  const bool has1 = l1 < a.nlines;
+  const bool has1 = l1 < a.line0 + a.nlines;
   const int j0 = l0 / a.nrows, r0 = l0 - j0 * a.nrows;
   const int j1 = has1 ? l1 / a.nrows : 0, r1 = has1 ? l1 - j1 * a.nrows : 0;
 
@@ -368,8 +411,8 @@ void rebin_filter_reg_kernel(RebinFilterArgs a) {
 
   const int npairs = (a.nlines + 1) >> 1;
   for (int pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
-    const int l0 = 2 * pair, l1 = l0 + 1;
-    const bool has1 = l1 < a.nlines;
+    const int l0 = a.line0 + 2 * pair, l1 = l0 + 1;
+    const bool has1 = l1 < a.line0 + a.nlines;
     const int j0 = l0 / a.nrows, r0 = l0 - j0 * a.nrows;
     const int j1 = has1 ? l1 / a.nrows : 0, r1 = has1 ? l1 - j1 * a.nrows : 0;
     FctCplx v[8];
@@ -516,19 +559,55 @@ hipError_t launch_reg(const RebinFilterArgs& a, hipStream_t s) {
 
 }  // namespace
 
+hipError_t launch_scale_reset(float* scale, hipStream_t s) {
+  hipLaunchKernelGGL(scale_reset_kernel, dim3(1), dim3(64), 0, s, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_set(float* scale, float max_abs_raw, hipStream_t s) {
+  hipLaunchKernelGGL(scale_set_kernel, dim3(1), dim3(64), 0, s, scale, max_abs_raw);
+  return hipGetLastError();
+}
+
+hipError_t launch_absmax(const float* x, int64_t n, float* scale, int ncu, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t want = (n / 4 + 255) / 256;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 8LL * std::max(ncu, 1)));
+  if ((reinterpret_cast<uintptr_t>(x) & 15) == 0)
+    hipLaunchKernelGGL(absmax_kernel<true>, dim3(grid), dim3(256), 0, s, x, n, scale);
+  else
+    hipLaunchKernelGGL(absmax_kernel<false>, dim3(grid), dim3(256), 0, s, x, n, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_update(float* scale, float hsum, hipStream_t s) {
+  hipLaunchKernelGGL(scale_update_kernel, dim3(1), dim3(64), 0, s, scale, hsum);
+  return hipGetLastError();
+}
+
+hipError_t launch_quad_rescale(void* quads, size_t n_entries, int precision, const float* scale,
+                               int ncu, hipStream_t s) {
+  if (n_entries == 0) return hipSuccess;
+  const size_t want = (n_entries + 255) / 256;
+  const int grid = (int)std::max<size_t>(1, std::min<size_t>(want, 8 * (size_t)std::max(ncu, 1)));
+  if (precision == FCT_PRECISION_FP16)
+    hipLaunchKernelGGL(quad_rescale_kernel<__half>, dim3(grid), dim3(256), 0, s,
+                       static_cast<uint2*>(quads), n_entries, scale);
+  else if (precision == FCT_PRECISION_BF16)
+    hipLaunchKernelGGL(quad_rescale_kernel<__hip_bfloat16>, dim3(grid), dim3(256), 0, s,
+                       static_cast<uint2*>(quads), n_entries, scale);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s) {
   if (a.nlines <= 0) return hipSuccess;
-  if (a.precision != FCT_PRECISION_FP32 && !a.rebin_only) {
+  if (a.line0 & 1) return hipErrorInvalidValue;
+  if (a.precision != FCT_PRECISION_FP32 && !a.rebin_only && !a.skip_scale) {
     if (!a.scale || a.lines_in) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(scale_reset_kernel, dim3(1), dim3(64), 0, s, a.scale);
-    const int64_t n = a.raw_elems;
-    const int64_t want = (n / 4 + 255) / 256;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 8LL * std::max(a.ncu, 1)));
-    if ((reinterpret_cast<uintptr_t>(a.raw) & 15) == 0)
-      hipLaunchKernelGGL(absmax_kernel<true>, dim3(grid), dim3(256), 0, s, a.raw, n, a.scale);
-    else
-      hipLaunchKernelGGL(absmax_kernel<false>, dim3(grid), dim3(256), 0, s, a.raw, n, a.scale);
-    const hipError_t e = hipGetLastError();
+    hipError_t e = launch_scale_reset(a.scale, s);
+    if (e == hipSuccess) e = launch_absmax(a.raw, a.raw_elems, a.scale, a.ncu, s);
     if (e != hipSuccess) return e;
   }
   switch (a.nfft) {

@@ -31,7 +31,9 @@ struct RebinDev {
 //    beyond the detector (c + 1 = cpitch, r + 1 = nrows) hold 0.  s is one
 //    power of two per scan, chosen from a bound on |y| so that |y| / s < 2^15
 //    (no overflow); it is kept in scale[0] of the scan's scale buffer.
-constexpr int kScaleElems = 2;   // [0] s, [1] max |raw| bits (rebin+filter scratch)
+// [0] s, [1] max |raw| bits (rebin+filter scratch), [2] rescale factor of the
+// streaming reconstruction, [3] unused.
+constexpr int kScaleElems = 4;
 
 // Channel weight of the FP16 tap interpolation: the fraction wp rounded to a
 // multiple of 2^-11, so that it and 1 - it are exact FP16 values summing to 1
@@ -59,6 +61,11 @@ struct RebinFilterArgs {
   int32_t states;            // focal states S (1 without FFS)
   int32_t zshift;            // 1 with z FFS: effective row q = 2 r + parity
   int32_t rebin_only;        // 1: write the rebinned rows (FP32) without filtering
+  // Lines [line0, line0 + nlines) of the scan (line0 even, so the line pairs
+  // are those of a whole-scan launch); the streaming reconstruction filters
+  // views as they arrive.
+  int32_t line0;
+  int32_t skip_scale;        // 1: the FP16 / BF16 bound is already in scale[1]
 };
 
 // Everything the backprojection kernel reads.
@@ -86,6 +93,10 @@ struct BpArgs {
   int32_t nx, ny, n_native, nviews, H, cpitch;
   int32_t tiles_x, tiles_y, nslabs;
   int32_t sel_ix, sel_iy, sel_is;
+  // Slabs [slab0, slab0 + slab_count) of a launch (slab_count 0: all); the
+  // streaming reconstruction backprojects groups of slabs as their views
+  // arrive.  Unused by the persistent batch kernel.
+  int32_t slab0, slab_count;
 };
 
 // Persistent multi-job backprojection (backproject_v1.hip).
@@ -135,6 +146,16 @@ struct RebinAdjArgs {
 };
 
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s);
+// FP16 / BF16 storage bound in scale[1]: cleared, raised to max|x| over n
+// floats (accumulating over calls), or set from a caller's max|raw|.
+hipError_t launch_scale_reset(float* scale, hipStream_t s);
+hipError_t launch_absmax(const float* x, int64_t n, float* scale, int ncu, hipStream_t s);
+hipError_t launch_scale_set(float* scale, float max_abs_raw, hipStream_t s);
+// Streaming FP16 / BF16: scale[0] from the bound so far, scale[2] the factor
+// for the quads already stored; then those n_entries quads rescaled by it.
+hipError_t launch_scale_update(float* scale, float hsum, hipStream_t s);
+hipError_t launch_quad_rescale(void* quads, size_t n_entries, int precision, const float* scale,
+                               int ncu, hipStream_t s);
 hipError_t launch_backproject_adjoint(const BpAdjArgs& a, int tile_z, hipStream_t s,
                                       bool matched = false);
 hipError_t launch_rebin_adjoint(const RebinAdjArgs& a, hipStream_t s);
@@ -215,6 +236,16 @@ struct fct_wfbp_gpu_plan {
   int32_t* d_thkT_off = nullptr;    // transposed thickening table
   int32_t* d_thkT_idx = nullptr;
   float* d_thkT_w = nullptr;
+
+  // Streaming reconstruction (fct_wfbp_gpu_reconstruct_stream and the host
+  // path of fct_wfbp_gpu_reconstruct), created on first use.
+  hipStream_t copy_stream = nullptr;     // H2D copies (default priority)
+  hipEvent_t ev_stage[2] = {nullptr, nullptr};
+  hipEvent_t ev_compute = nullptr;
+  float* h_stage[2] = {nullptr, nullptr};   // pinned staging for a view reader
+  int stage_views = 0;
+  std::vector<int2> h_slab_vrange;      // host copy of d_slab_vrange
+  int rebin_reach = 0;                  // parallel view j reads raw projections < j S + reach
 
   int64_t raw_elems() const {
     return (int64_t)raw_scan.n_views * raw_scan.n_rows * raw_scan.n_channels;

@@ -87,6 +87,7 @@ SIGNATURES = {
     "fct_wfbp_gpu_backproject": (C.c_int, [_VP, _VP, _VP, _VP]),
     "fct_wfbp_gpu_thicken": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_reconstruct": (C.c_int, [_VP, _VP, _VP, C.c_int32]),
+    "fct_wfbp_gpu_reconstruct_stream": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_float, _VP]),
     "fct_wfbp_gpu_backproject_adjoint": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_filter": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_rebin_adjoint": (C.c_int, [_VP, _VP, _VP]),
@@ -386,11 +387,65 @@ class Plan:
         self.ctx.check(load_library().fct_wfbp_gpu_reconstruct(
             self.h, self._raw(raw), self._volume(volume), 0), "reconstruct")
 
-    def reconstruct_host(self, raw: np.ndarray) -> np.ndarray:
-        raw = np.ascontiguousarray(raw, np.float32)
-        vol = np.empty((self.info.n_slices_requested, self.recon.ny, self.recon.nx), np.float32)
+    def _host_volume(self, out):
+        shape = (self.info.n_slices_requested, self.recon.ny, self.recon.nx)
+        if out is None:
+            return np.empty(shape, np.float32)
+        if (not isinstance(out, np.ndarray) or out.dtype != np.float32 or out.shape != shape
+                or not out.flags["C_CONTIGUOUS"]):
+            raise FctError(f"volume: expected a C-contiguous float32 array of shape {shape}")
+        return out
+
+    def reconstruct_host(self, raw, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """Whole chain from a host scan (numpy array, or a CPU torch tensor --
+        pinned memory makes the chunked copies asynchronous DMA) to a host
+        volume, streamed in chunks of views (fct_wfbp_gpu_reconstruct with
+        host_buffers = 1)."""
+        if hasattr(raw, "data_ptr"):   # torch CPU tensor
+            if raw.is_cuda or raw.dtype.itemsize != 4 or not raw.is_contiguous() or \
+                    str(raw.dtype) != "torch.float32":
+                raise FctError("raw: expected a contiguous float32 CPU tensor")
+            if raw.numel() != self.info.raw_elems:
+                raise FctError(f"raw: expected {self.info.raw_elems} elements")
+            ptr, keep = raw.data_ptr(), raw
+        else:
+            keep = np.ascontiguousarray(raw, np.float32)
+            if keep.size != self.info.raw_elems:
+                raise FctError(f"raw: expected {self.info.raw_elems} elements, got {keep.size}")
+            ptr = keep.ctypes.data
+        vol = self._host_volume(out)
         self.ctx.check(load_library().fct_wfbp_gpu_reconstruct(
-            self.h, raw.ctypes.data, vol.ctypes.data, 1), "reconstruct(host)")
+            self.h, ptr, vol.ctypes.data, 1), "reconstruct(host)")
+        del keep
+        return vol
+
+    READER = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_float))
+
+    def reconstruct_stream(self, read_views, views_per_chunk: int = 0, max_abs_raw: float = 0.0,
+                           out: Optional[np.ndarray] = None) -> np.ndarray:
+        """Streaming whole chain (fct_wfbp_gpu_reconstruct_stream):
+        read_views(v0, n, dst) fills views [v0, v0 + n) into dst, a float32
+        numpy view of the plan's pinned staging, shape (n, rows, channels)."""
+        rows, chans = self.scan.n_rows, self.scan.n_channels
+        errors = []
+
+        def _cb(_user, v0, n, dst):
+            try:
+                arr = np.ctypeslib.as_array(dst, shape=(n * rows * chans,)).reshape(n, rows, chans)
+                read_views(int(v0), int(n), arr)
+                return 0
+            except Exception as e:   # reported after the call
+                errors.append(e)
+                return 1
+
+        cb = self.READER(_cb)
+        vol = self._host_volume(out)
+        st = load_library().fct_wfbp_gpu_reconstruct_stream(
+            self.h, C.cast(cb, C.c_void_p), None, int(views_per_chunk), float(max_abs_raw),
+            vol.ctypes.data)
+        if errors:
+            raise errors[0]
+        self.ctx.check(st, "reconstruct_stream")
         return vol
 
     # -- adjoint (transpose of rebin -> filter -> backproject, FP32) ----------

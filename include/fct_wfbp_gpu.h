@@ -193,10 +193,40 @@ FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* plan, const float* native, f
 
 /* Whole chain (RunReconstruction, ReconstructionMachine.cu:76-82).  raw and
  * volume are device pointers unless host_buffers != 0, in which case they are
- * host pointers and the call returns after the volume is copied back.  The
- * plan owns the intermediate buffers. */
+ * host pointers, the scan is streamed to the device in chunks of views as in
+ * fct_wfbp_gpu_reconstruct_stream (copies on the plan's copy stream,
+ * overlapped with rebin+filter and backprojection of what has arrived; a
+ * pinned raw array, e.g. from hipHostMalloc, makes the copies asynchronous
+ * DMA) and the call returns after the volume is copied back.  The plan owns
+ * the intermediate buffers. */
 FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* plan, const float* raw, float* volume,
                                      int32_t host_buffers);
+
+/* Source of raw views for the streaming chain: writes views [v0, v0 + n) of
+ * the scan, [view][row][ch] float32, to dst (pinned host memory owned by the
+ * plan) and returns 0, or nonzero to abort.  Called in view order on the
+ * thread that called fct_wfbp_gpu_reconstruct_stream, while the previous
+ * chunk is being copied and processed.  It takes the place of
+ * RawDataSet::copyProjection (RawDataSet.h:136-139) as AllocateKeyArrays
+ * uses it (ReconstructionMachine.cu:88-93). */
+typedef int (*fct_view_reader)(void* user, int32_t v0, int32_t n, float* dst);
+
+/* Streaming whole chain: RunReconstruction with the reference's host copy of
+ * every frame, its single-threaded flip and its synchronous per-row copies
+ * (ReconstructionMachine.cu:88-93, 150-162, 246) replaced by chunked,
+ * double-buffered pinned staging and asynchronous H2D on a copy stream.  On
+ * the plan's stream the parallel views whose raw views have all arrived are
+ * rebinned and filtered, and groups of slabs whose views are all filtered are
+ * backprojected, while later chunks are read and copied.  views_per_chunk: 0
+ * for 256.  max_abs_raw: for FP16/BF16 plans a bound >= max|raw| fixes the
+ * storage scale up front (an underestimate overflows the 16-bit values); 0
+ * uses the running abs-max of the views so far and rescales the stored
+ * projections by a power of two when it grows.  The volume (same values as
+ * fct_wfbp_gpu_reconstruct) is copied to volume_host [slice][y][x] before the
+ * call returns. */
+FCT_API int fct_wfbp_gpu_reconstruct_stream(fct_wfbp_gpu_plan* plan, fct_view_reader read,
+                                            void* user, int32_t views_per_chunk,
+                                            float max_abs_raw, float* volume_host);
 
 /* Adjoint (no reference counterpart): the exact transpose of the FP32 chain
  * rebin -> filter -> backproject, i.e. a matched helical forward projector.

@@ -570,8 +570,8 @@ void or_view_range(const or_scan* s, const or_recon* r, int s0, int s1, int* v0,
   else if (G.dz < 0) { a = (hi - s->z0) / G.dz; b = (lo - s->z0) / G.dz; }
   else { a = 0; b = G.nviews; }
   int lo_v = (int)std::floor(a) - 2, hi_v = (int)std::ceil(b) + 3;
-  *v0 = std::max(0, lo_v);
-  *v1 = std::min(G.nviews, hi_v);
+  *v0 = std::max(0, std::min(lo_v, G.nviews));
+  *v1 = std::max(*v0, std::min(G.nviews, hi_v));   // empty when the slices lie past the scan
 }
 
 int or_reconstruct_native(const or_scan* raw_scan, const or_recon* r, const float* raw,
@@ -581,6 +581,13 @@ int or_reconstruct_native(const or_scan* raw_scan, const or_recon* r, const floa
   const Geo G = make_geo(s);
   int v0, v1;
   or_view_range(s, r, s0, s1, &v0, &v1);
+  if (v1 <= v0) {   // no view reaches these slices
+    or_derived d;
+    if (or_geometry(s, r, &d) != 0 || s0 < 0 || s1 > d.n_slices_native || s0 >= s1) return -1;
+    std::fill(vol, vol + (size_t)(s1 - s0) * r->nx * r->ny, 0.0);
+    if (n_selected) *n_selected = 0;
+    return 0;
+  }
   const size_t vs = (size_t)G.nrows * G.ncols;
   std::vector<double> par((size_t)(v1 - v0) * vs);
   or_rebin(raw_scan, raw, par.data(), v0, v1);

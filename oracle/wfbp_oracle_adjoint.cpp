@@ -185,6 +185,11 @@ int or_adjoint_native(const or_scan* raw_scan, const or_recon* r, const double* 
   or_view_range(s, r, s0, s1, &v0, &v1);
   or_derived d;
   or_geometry(s, r, &d);
+  if (v1 <= v0) {   // no view reaches these slices: the transpose is zero
+    std::fill(pbar, pbar + (size_t)raw_scan->n_views * raw_scan->n_rows * raw_scan->n_channels,
+              0.0);
+    return 0;
+  }
   const size_t vs = (size_t)s->n_rows * s->n_channels;
   std::vector<double> gbar((size_t)(v1 - v0) * vs, 0.0);
   int st = or_backproject_adjoint(s, r, xbar, v0, v1, s0, s1, gbar.data());

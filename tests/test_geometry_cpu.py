@@ -221,3 +221,17 @@ def test_anode_angle_recon_key(tmp_path):
                        timeout=60)
     assert r.returncode == 0
     assert abs(json.loads(r.stdout)["anode_angle"] - 0.15) < 1e-7
+
+
+def test_oracle_slices_past_the_scan_are_zero(oracle):
+    # Native slices no view reaches: the restatement returns zeros (an empty
+    # view range), forward and transposed.
+    w = S.tiny()
+    rc = w.recon.replace(start_pos=-400.0, end_pos=-380.0)
+    raw = oracle.project_phantom(w.scan, w.phantom)
+    a, b = oracle.view_range(w.scan, rc, 0, 1)
+    assert a == b
+    vol, n = oracle.reconstruct_native(w.scan, rc, raw, 0, 2)
+    assert n == 0 and not vol.any()
+    pbar = oracle.adjoint_native(w.scan, rc, np.ones((2, rc.ny, rc.nx)), 0)
+    assert not pbar.any()

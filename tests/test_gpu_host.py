@@ -71,3 +71,85 @@ def test_cli_freect_format_round_trip(ctx, tmp_path):
     d = np.abs(got.astype(np.float64) - ref) / (S.MU_WATER / 1000.0)
     say("freect format round trip: max %.4f HU, mean %.5f HU" % (d.max(), d.mean()))
     assert d.mean() < 0.05 and np.percentile(d, 99.9) < 1.0
+
+
+def _write_freect(dirpath, sc, raw, angles, zs, ffs="FFSNONE"):
+    meta = {"manufacturer": "SYNTHETIC", "detector_rows": sc.n_rows,
+            "detector_channels": sc.n_channels,
+            "detector_transverse_spacing": sc.channel_spacing,
+            "detector_axial_spacing": sc.row_spacing, "detector_shape": "CYLINDRICAL",
+            "distance_source_to_detector": sc.dsd, "distance_source_to_isocenter": sc.r,
+            "detector_central_row": sc.central_row,
+            "detector_central_channel": sc.central_channel, "projection_geometry": "FANBEAM",
+            "scan_type": "HELICAL", "projections_per_rotation": sc.views_per_rotation,
+            "flying_focal_spot_mode": ffs, "total_num_projections": raw.shape[0]}
+    with open(os.path.join(dirpath, "meta.yaml"), "w") as f:
+        for k, v in meta.items():
+            f.write(f"{k}: {v!r}\n" if isinstance(v, float) else f"{k}: {v}\n")
+    with open(os.path.join(dirpath, "projections.dat"), "wb") as f:
+        for v in range(raw.shape[0]):
+            np.array([angles[v], zs[v], sc.r], np.float32).tofile(f)
+            raw[v].astype(np.float32).tofile(f)
+
+
+@pytest.mark.parametrize("case", ["ccw", "cw", "ffs"])
+def test_cli_freect_file_matches_oracle(ctx, oracle, tmp_path, case):
+    # A FreeCT two-file scan (RawDataSet.cpp:85-107) written here from the
+    # FP64 projector, read by the command-line driver (frame headers first,
+    # the projections streamed from disk into the chunked GPU upload) and
+    # reconstructed; the FP64 restatement reconstructs the same float32 data
+    # on the geometry the driver derives from the headers.  Counter-clockwise,
+    # clockwise, and phi+z flying focal spot with per-state deflections in the
+    # frame headers.
+    import json
+    if case == "ffs":
+        w = S.tiny(nx=48, n_slices=12, n_channels=96, n_rows=8, ppr=192, rotations=3,
+                   ffs_mode=3)
+    else:
+        w = S.tiny(nx=48, n_slices=12, n_channels=192, n_rows=16, ppr=192, rotations=3)
+    sc = w.scan.replace(rotation=-1, angle0=1.1) if case == "cw" else w.scan
+    rc = w.recon
+    raw = oracle.project_phantom(sc, w.phantom)
+    v = np.arange(sc.n_views)
+    angles = sc.angle0 + sc.rotation * v * 2 * np.pi / sc.views_per_rotation
+    zs = sc.z0 + v * sc.feed / sc.views_per_rotation
+    if case == "ffs":   # per focal state deflection offsets, mean zero
+        angles = angles + np.array([3e-4, -3e-4, 3e-4, -3e-4])[v % 4]
+        zs = zs + np.array([0.05, 0.05, -0.05, -0.05])[v % 4]
+    _write_freect(str(tmp_path), sc, raw, angles, zs, "FFSXYZ" if case == "ffs" else "FFSNONE")
+    cfg = tmp_path / "recon.yaml"
+    cfg.write_text(f"raw_data_dir: {tmp_path}\noutput_dir: {tmp_path}\nstart_pos: {rc.start_pos!r}\n"
+                   f"end_pos: {rc.end_pos!r}\nrecon_fov: {rc.recon_fov!r}\n"
+                   f"slice_thickness: {rc.slice_thickness!r}\nslice_pitch: {rc.slice_pitch!r}\n"
+                   f"nx: {rc.nx}\nny: {rc.ny}\nfilter: {rc.filter}\n"
+                   f"anode_angle: {sc.anode_angle!r}\n")
+    out = tmp_path / "vol.dat"
+    say("cli freect file", case)
+    r = subprocess.run([CLI, "-o", str(out), str(cfg)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout
+    r = subprocess.run([CLI, "--inspect", str(tmp_path)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout
+    g = json.loads(r.stdout)
+    assert g["rotation"] == sc.rotation and g["ffs_mode"] == sc.ffs_mode
+    osc = sc.replace(channel_spacing=g["channel_spacing"], row_spacing=g["row_spacing"],
+                     dsd=g["dsd"], r=g["r"], central_channel=g["central_channel"],
+                     central_row=g["central_row"], angle0=g["angle0"], z0=g["z0"],
+                     feed=g["feed"])
+    # The recon keys pass through float32 (recon_structs.h), as in the driver.
+    f32 = lambda x: float(np.float32(x))   # noqa: E731
+    orc = rc.replace(start_pos=f32(rc.start_pos), end_pos=f32(rc.end_pos),
+                     recon_fov=f32(rc.recon_fov), slice_thickness=f32(rc.slice_thickness),
+                     slice_pitch=f32(rc.slice_pitch))
+    d = oracle.geometry(osc, orc)
+    got = np.fromfile(out, np.float32).reshape(d.n_slices_requested, rc.ny, rc.nx)
+    nat, _ = oracle.reconstruct_native(osc, orc, raw, 0, d.n_slices_native)
+    ref = oracle.thicken(osc, orc, nat)
+    from helpers import fov_mask
+    cov = oracle.covered_mask(osc, orc) & fov_mask(orc, d.acq_fov)[None]
+    dd = (got.astype(np.float64) - ref)[cov] / (S.MU_WATER / 1000.0)
+    rmse, mx = float(np.sqrt(np.mean(dd * dd))), float(np.abs(dd).max())
+    say("cli freect %s vs fp64: covered %d, rmse %.4f max %.4f HU" % (case, cov.sum(), rmse, mx))
+    assert cov.sum() > 0.2 * cov.size
+    assert rmse <= 0.5 and mx <= 2.0
