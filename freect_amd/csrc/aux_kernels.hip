@@ -15,6 +15,9 @@ __global__ __launch_bounds__(256) void thicken_kernel(ThickenArgs a) {
   float acc = 0.0f;
   for (int k = a.off[s]; k < a.off[s + 1]; ++k)
     acc = fmaf(a.w[k], a.native[(size_t)a.idx[k] * a.plane + p], acc);
+  // Hounsfield units on request (the reference stores mm^-1 only,
+  // ReconstructionMachine.cu:436-455).
+  if (a.hu) acc = fmaf(acc, a.hu_scale, -1000.0f);
   a.out[(size_t)s * a.plane + p] = acc;
 }
 

@@ -137,6 +137,11 @@ int validate(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* s, const fct_recon_
     return set_err(ctx, "recon_fov and slice_thickness must be > 0"), FCT_ERR_INVALID_ARGUMENT;
   if (r->filter_kind < 0 || r->filter_kind > 1)
     return set_err(ctx, "unknown filter_kind %d", r->filter_kind), FCT_ERR_INVALID_ARGUMENT;
+  if (o && o->adaptive_filtration &&
+      !(r->adaptive_filtration_s > 0.0 && r->adaptive_filtration_s <= 1.0))
+    return set_err(ctx, "adaptive filtration needs adaptive_filtration_s in (0, 1] (got %g)",
+                   r->adaptive_filtration_s),
+           FCT_ERR_INVALID_ARGUMENT;
   if (o) {
     if (o->precision < 0 || o->precision > 2)
       return set_err(ctx, "unknown precision %d", o->precision), FCT_ERR_INVALID_ARGUMENT;
@@ -153,7 +158,7 @@ void free_plan_buffers(fct_wfbp_gpu_plan* p) {
                   p->d_slab_vrange, p->d_rebin,  p->d_Hn,        p->d_tw,       p->d_thk_off,
                   p->d_thk_idx,  p->d_thk_w,     p->d_filtered,  p->d_scale, p->d_native,
                   p->d_raw_stage, p->d_vol_stage, p->d_gbar, p->d_thkT_off, p->d_thkT_idx,
-                  p->d_thkT_w, p->d_wrange, p->d_zbounds};
+                  p->d_thkT_w, p->d_wrange, p->d_zbounds, p->d_af_raw, p->d_af_vmax};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int b = 0; b < 2; ++b) {
@@ -346,7 +351,9 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
       HIP_TRY(ctx, fct::launch_scale_update(p->d_scale, (float)p->hsum, p->stream));
     }
   }
-  fct::RebinFilterArgs rf = rf_args(p, p->d_raw_stage, p->d_filtered, p->d_scale);
+  // With adaptive filtration the rebinning reads the filtered copy of each chunk.
+  const float* rebin_src = p->af_on ? p->d_af_raw : p->d_raw_stage;
+  fct::RebinFilterArgs rf = rf_args(p, rebin_src, p->d_filtered, p->d_scale);
   rf.skip_scale = 1;
   fct::BpArgs bp = bp_args(p);
   bp.filt = p->d_filtered;
@@ -416,8 +423,13 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
     if (e == hipSuccess) e = hipEventRecord(p->ev_stage[b], p->copy_stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(p->stream, p->ev_stage[b], 0);
     const int arrived = v0 + n;
+    if (e == hipSuccess && p->af_on)
+      e = fct::launch_adaptive_filter(p->d_raw_stage, p->d_af_raw, p->d_af_vmax, v0, arrived,
+                                      p->raw_scan.n_rows, p->raw_scan.n_channels, p->af_s,
+                                      p->stream);
     if (e == hipSuccess && grow)
-      e = fct::launch_absmax(dst, (int64_t)n * (int64_t)per_view, p->d_scale, p->ncu, p->stream);
+      e = fct::launch_absmax(rebin_src + (size_t)v0 * per_view, (int64_t)n * (int64_t)per_view,
+                             p->d_scale, p->ncu, p->stream);
     // Parallel views j with j S + reach <= arrived have every raw view they read.
     int j1 = npar;
     if (arrived < nv) {
@@ -519,6 +531,8 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   p->scan.ffs_mode = FCT_FFS_NONE;
   p->recon = *recon;
   if (opts) p->opts = *opts;
+  p->af_on = opts && opts->adaptive_filtration ? 1 : 0;
+  p->af_s = (float)recon->adaptive_filtration_s;
   p->stream = static_cast<hipStream_t>(stream);   // NULL: the null stream
   if (hipDeviceGetAttribute(&p->ncu, hipDeviceAttributeMultiprocessorCount, p->device) !=
       hipSuccess)
@@ -716,6 +730,12 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   PLAN_TRY(upload(&p->d_zt, zt, p->stream));
   PLAN_TRY(upload(&p->d_slab_vrange, vr, p->stream));
   p->h_slab_vrange = vr;
+  if (p->af_on) {   // at set-up, so that no stage call (or captured graph) allocates
+    PLAN_TRY(hipMalloc(reinterpret_cast<void**>(&p->d_af_raw),
+                       (size_t)p->raw_elems() * sizeof(float)));
+    PLAN_TRY(hipMalloc(reinterpret_cast<void**>(&p->d_af_vmax),
+                       (size_t)p->raw_scan.n_views * sizeof(float)));
+  }
   PLAN_TRY(upload(&p->d_wrange, wrange, p->stream));
   PLAN_TRY(upload(&p->d_rebin, tab, p->stream));
   PLAN_TRY(upload(&p->d_Hn, Hn, p->stream));
@@ -789,6 +809,12 @@ FCT_API int fct_wfbp_gpu_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, vo
     return set_err(p->ctx, "the scale buffer is required for FP16/BF16"),
            FCT_ERR_INVALID_ARGUMENT;
   HIP_TRY(p->ctx, hipSetDevice(p->device));
+  if (p->af_on) {   // adaptive filtration of the whole scan into the plan's buffer
+    HIP_TRY(p->ctx, fct::launch_adaptive_filter(raw, p->d_af_raw, p->d_af_vmax, 0,
+                                                p->raw_scan.n_views, p->raw_scan.n_rows,
+                                                p->raw_scan.n_channels, p->af_s, p->stream));
+    raw = p->d_af_raw;
+  }
   HIP_TRY(p->ctx, fct::launch_rebin_filter(rf_args(p, raw, filtered, scale), p->stream));
   return FCT_OK;
 }
@@ -820,7 +846,23 @@ FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* p, const float* native, floa
   a.w = p->d_thk_w;
   a.plane = p->recon.nx * p->recon.ny;
   a.n_req = p->n_req;
+  a.hu = p->out_hu;
+  a.hu_scale = p->out_hu ? (float)(1000.0 / p->mu_water) : 1.0f;
   HIP_TRY(p->ctx, fct::launch_thicken(a, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_set_output_units(fct_wfbp_gpu_plan* p, int32_t units, double mu_water) {
+  if (!p) return FCT_ERR_INVALID_ARGUMENT;
+  if (units == FCT_UNITS_MU) {
+    p->out_hu = 0;
+    return FCT_OK;
+  }
+  if (units != FCT_UNITS_HU) return set_err(p->ctx, "unknown units %d", units), FCT_ERR_INVALID_ARGUMENT;
+  if (!(mu_water > 0.0) || !(mu_water < 1e3))
+    return set_err(p->ctx, "mu_water must be > 0 (1/mm)"), FCT_ERR_INVALID_ARGUMENT;
+  p->out_hu = 1;
+  p->mu_water = mu_water;
   return FCT_OK;
 }
 

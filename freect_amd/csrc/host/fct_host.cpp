@@ -151,7 +151,9 @@ int parse_config(const std::string& config_file, ReconConfig& rp, bool verbose, 
             get("recon_kernel", rp.recon_kernel) && get("x_origin", rp.x_origin) &&
             get("y_origin", rp.y_origin) && get("tube_angle_offset", rp.tube_angle_offset) &&
             get("adaptive_filtration_s", rp.adaptive_filtration_s) && get("filter", rp.filter) &&
-            get("precision", rp.precision) && get("anode_angle", rp.anode_angle);
+            get("precision", rp.precision) && get("anode_angle", rp.anode_angle) &&
+            get("adaptive_filtration", rp.adaptive_filtration) &&
+            get("output_units", rp.output_units) && get("mu_water", rp.mu_water);
   if (!ok) return FCT_ERR_INVALID_ARGUMENT;
   rp.output_file = config_file + "_recon.dat";        // parse_config.cpp:43
   if (rp.slice_pitch == 0) rp.slice_pitch = rp.slice_thickness;   // parse_config.cpp:59-60
@@ -848,6 +850,10 @@ int ReconstructionMachine::RunReconstruction() {
     m_err = "unknown precision '" + m_rp.precision + "'";
     return FCT_ERR_INVALID_ARGUMENT;
   }
+  if (m_rp.output_units != "mu" && m_rp.output_units != "hu") {
+    m_err = "unknown output_units '" + m_rp.output_units + "' (mu or hu)";
+    return FCT_ERR_INVALID_ARGUMENT;
+  }
   // AllocateKeyArrays copies every frame into one [view][row][ch] host array
   // (ReconstructionMachine.cu:84-105); here the data set's streaming
   // iterator fills the plan's pinned staging chunk by chunk while earlier
@@ -860,8 +866,11 @@ int ReconstructionMachine::RunReconstruction() {
   }
   fct_exec_options opts{};
   opts.precision = prec;
+  opts.adaptive_filtration = m_rp.adaptive_filtration ? 1 : 0;
   fct_wfbp_gpu_plan* plan = nullptr;
   st = fct_wfbp_gpu_plan_create(ctx, &m_geom, &m_rc, &opts, nullptr, &plan);
+  if (!st && m_rp.output_units == "hu")
+    st = fct_wfbp_gpu_set_output_units(plan, FCT_UNITS_HU, m_rp.mu_water);
   if (!st) st = fct_wfbp_gpu_plan_info(plan, &m_info);
   if (!st) {
     m_reconstructed_data.assign((size_t)m_info.volume_elems, 0.0f);
@@ -900,7 +909,12 @@ int ReconstructionMachine::RunReconstruction() {
 }
 
 // ReconstructionMachine.cu:436-455; an empty path means output_dir/output_file
-// (the reference's empty-path branch writes to an empty path).
+// (the reference's empty-path branch writes to an empty path).  The volume
+// file itself is the reference's: headerless little-endian float32
+// [slice][y][x].  Beside it goes a detached NRRD header (<file>.nhdr; NRRD
+// format version 4, readable by ITK / 3D Slicer / teem) that names the file,
+// its sizes, voxel spacings and the centre of the first voxel, and carries
+// the reconstruction's own keys (units, slice thickness, recon config).
 int ReconstructionMachine::SaveReconstruction(const std::string& filepath) {
   std::string path = filepath;
   if (path.empty()) path = (m_rp.output_dir.empty() ? "." : m_rp.output_dir) + "/" + m_rp.output_file;
@@ -912,7 +926,52 @@ int ReconstructionMachine::SaveReconstruction(const std::string& filepath) {
   }
   o.write(reinterpret_cast<const char*>(m_reconstructed_data.data()),
           m_reconstructed_data.size() * sizeof(float));
-  return o ? FCT_OK : FCT_ERR_INVALID_ARGUMENT;
+  if (!o) {
+    m_err = "cannot write " + path;
+    return FCT_ERR_INVALID_ARGUMENT;
+  }
+  o.close();
+  return WriteSidecarHeader(path);
+}
+
+int ReconstructionMachine::WriteSidecarHeader(const std::string& volume_path) const {
+  const std::string hdr = volume_path + ".nhdr";
+  std::ofstream h(hdr);
+  if (!h) return FCT_ERR_INVALID_ARGUMENT;
+  const double dx = m_rc.recon_fov / m_rc.nx, dy = m_rc.recon_fov / m_rc.ny;
+  const double pitch = m_rc.slice_pitch > 0 ? m_rc.slice_pitch : m_rc.slice_thickness;
+  const double dz = m_rc.end_pos >= m_rc.start_pos ? pitch : -pitch;   // slice step (signed)
+  const double x0 = -0.5 * (m_rc.nx - 1) * dx + m_rc.x_origin;
+  const double y0 = -0.5 * (m_rc.ny - 1) * dy + m_rc.y_origin;
+  std::string base = volume_path;
+  const size_t slash = base.find_last_of('/');
+  if (slash != std::string::npos) base = base.substr(slash + 1);
+  const bool hu = m_rp.output_units == "hu";
+  h << std::setprecision(12);
+  h << "NRRD0004\n"
+    << "# FreeCT WFBP reconstruction: headerless float32 [slice][y][x], x fastest\n"
+    << "type: float\n"
+    << "dimension: 3\n"
+    << "sizes: " << m_rc.nx << " " << m_rc.ny << " " << m_info.n_slices_requested << "\n"
+    << "spacings: " << dx << " " << dy << " " << dz << "\n"
+    << "axis mins: " << x0 << " " << y0 << " " << m_rc.start_pos << "\n"
+    << "units: \"mm\" \"mm\" \"mm\"\n"
+    << "kinds: domain domain domain\n"
+    << "endian: little\n"
+    << "encoding: raw\n"
+    << "data file: " << base << "\n"
+    << "freect_units:=" << (hu ? "HU" : "1/mm") << "\n";
+  if (hu) h << "freect_mu_water:=" << m_rp.mu_water << "\n";
+  h << "freect_slice_thickness:=" << m_rc.slice_thickness << "\n"
+    << "freect_start_pos:=" << m_rc.start_pos << "\n"
+    << "freect_end_pos:=" << m_rc.end_pos << "\n"
+    << "freect_recon_fov:=" << m_rc.recon_fov << "\n"
+    << "freect_filter:=" << m_rp.filter << "\n"
+    << "freect_precision:=" << m_rp.precision << "\n"
+    << "freect_adaptive_filtration_s:="
+    << (m_rp.adaptive_filtration ? m_rp.adaptive_filtration_s : 0.0f) << "\n"
+    << "freect_native_slices:=" << m_info.n_slices_native << "\n";
+  return h ? FCT_OK : FCT_ERR_INVALID_ARGUMENT;
 }
 
 }  // namespace fct

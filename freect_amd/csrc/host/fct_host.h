@@ -38,6 +38,9 @@ struct ReconConfig {
   std::string filter = "ramp";     // ramp | sharp | medium | smooth | hann | hamming | shepp-logan
   std::string precision = "fp32";  // fp32 | fp16 | bf16
   float anode_angle = 0;           // rad, z-FFS deflection; 0 -> 7 degrees
+  int adaptive_filtration = 0;     // 1: apply adaptive_filtration_s (the reference ignores it)
+  std::string output_units = "mu";  // mu (1/mm, the reference) | hu
+  double mu_water = 0.0193;        // 1/mm, for output_units hu
 };
 
 // Flat "key: value" YAML maps (the only YAML the path reads or writes).
@@ -217,6 +220,8 @@ class ReconstructionMachine {
   // checked against every view (FCT_ERR_UNSUPPORTED when a view is off it).
   static int ViewLaw(const RawDataSet& ds, fct_scan_geometry* g, std::string* err);
   int SaveReconstruction(const std::string& filepath = "");
+  // <volume>.nhdr: detached NRRD header of the saved volume.
+  int WriteSidecarHeader(const std::string& volume_path) const;
 
   void setDevice(int device) { m_device = device; }
   void setVerbose(bool v) { m_verbose = v; }

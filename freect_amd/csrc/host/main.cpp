@@ -100,11 +100,13 @@ int main(int argc, char** argv) {
                 "\"slice_thickness\": %.9g, \"slice_pitch\": %.9g, \"nx\": %zu, \"ny\": %zu, "
                 "\"recon_kernel\": %d, \"x_origin\": %.9g, \"y_origin\": %.9g, "
                 "\"tube_angle_offset\": %.9g, \"adaptive_filtration_s\": %.9g, "
-                "\"filter\": \"%s\", \"precision\": \"%s\", \"anode_angle\": %.9g}\n",
+                "\"filter\": \"%s\", \"precision\": \"%s\", \"anode_angle\": %.9g, "
+                "\"adaptive_filtration\": %d, \"output_units\": \"%s\", \"mu_water\": %.9g}\n",
                 rp.raw_data_dir.c_str(), rp.output_dir.c_str(), rp.output_file.c_str(),
                 rp.start_pos, rp.end_pos, rp.recon_fov, rp.slice_thickness, rp.slice_pitch, rp.nx,
                 rp.ny, rp.recon_kernel, rp.x_origin, rp.y_origin, rp.tube_angle_offset,
-                rp.adaptive_filtration_s, rp.filter.c_str(), rp.precision.c_str(), rp.anode_angle);
+                rp.adaptive_filtration_s, rp.filter.c_str(), rp.precision.c_str(), rp.anode_angle,
+                rp.adaptive_filtration, rp.output_units.c_str(), rp.mu_water);
     return 0;
   }
 

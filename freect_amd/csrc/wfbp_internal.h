@@ -116,6 +116,8 @@ struct ThickenArgs {
   const float* w;            // normalised weights
   int32_t plane;             // nx*ny
   int32_t n_req;
+  int32_t hu;                // 1: store 1000 (mu - mu_water) / mu_water = mu hu_scale - 1000
+  float hu_scale;            // 1000 / mu_water
 };
 
 struct PhantomArgs {
@@ -146,6 +148,11 @@ struct RebinAdjArgs {
 };
 
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s);
+// Adaptive filtration of raw views [v0, v1): per view the largest finite
+// |p| (vmax), then every sample blended with its 3 x 3 (row, channel) mean by
+// lambda = clamp((p - (1 - s) vmax) / (s vmax), 0, 1).  out != in.
+hipError_t launch_adaptive_filter(const float* in, float* out, float* vmax, int v0, int v1,
+                                  int nrows, int ncols, float s, hipStream_t st);
 // FP16 / BF16 storage bound in scale[1]: cleared, raised to max|x| over n
 // floats (accumulating over calls), or set from a caller's max|raw|.
 hipError_t launch_scale_reset(float* scale, hipStream_t s);
@@ -209,6 +216,10 @@ struct fct_wfbp_gpu_plan {
   double dgamma = 0, ds = 0, dtheta = 0, dz = 0, csw = 0, acq = 0;
   FctBpConsts g{};
   float scale = 0;
+  int out_hu = 0;                   // thickened volume in HU (fct_wfbp_gpu_set_output_units)
+  double mu_water = 0;
+  int af_on = 0;                    // adaptive filtration of the raw scan before rebinning
+  float af_s = 0;                   // its strength s in (0, 1]
 
   // Device tables owned by the plan.
   float* d_xs = nullptr;
@@ -236,6 +247,8 @@ struct fct_wfbp_gpu_plan {
   int32_t* d_thkT_off = nullptr;    // transposed thickening table
   int32_t* d_thkT_idx = nullptr;
   float* d_thkT_w = nullptr;
+  float* d_af_raw = nullptr;        // adaptively filtered raw scan (allocated on first use)
+  float* d_af_vmax = nullptr;       // per raw view max |p|
 
   // Streaming reconstruction (fct_wfbp_gpu_reconstruct_stream and the host
   // path of fct_wfbp_gpu_reconstruct), created on first use.

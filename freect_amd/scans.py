@@ -127,6 +127,7 @@ class ReconConfig:
     y_origin: float = 0.0
     filter: str = "ramp"
     precision: str = "fp32"     # fp32 | fp16 (filtered projections storage)
+    adaptive_filtration_s: float = 0.0   # strength, used when a plan enables it
 
     @property
     def filter_kind(self):

@@ -48,7 +48,7 @@ class ReconConfigC(C.Structure):
 
 class ExecOptionsC(C.Structure):
     _fields_ = [("precision", C.c_int32), ("tile_z", C.c_int32), ("bp_variant", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("adaptive_filtration", C.c_int32)]
 
 
 class PlanInfoC(C.Structure):
@@ -88,6 +88,7 @@ SIGNATURES = {
     "fct_wfbp_gpu_thicken": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_reconstruct": (C.c_int, [_VP, _VP, _VP, C.c_int32]),
     "fct_wfbp_gpu_reconstruct_stream": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_float, _VP]),
+    "fct_wfbp_gpu_set_output_units": (C.c_int, [_VP, C.c_int32, C.c_double]),
     "fct_wfbp_gpu_backproject_adjoint": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_filter": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_rebin_adjoint": (C.c_int, [_VP, _VP, _VP]),
@@ -144,7 +145,8 @@ def scan_struct(scan: S.ScanGeometry) -> ScanGeometryC:
 def recon_struct(rc: S.ReconConfig) -> ReconConfigC:
     return ReconConfigC(rc.start_pos, rc.end_pos, rc.recon_fov, rc.slice_thickness,
                         rc.slice_pitch, rc.nx, rc.ny, rc.x_origin, rc.y_origin, 0,
-                        rc.filter_kind, rc.filter_c, rc.filter_a, 0.0, 0.0)
+                        rc.filter_kind, rc.filter_c, rc.filter_a, 0.0,
+                        getattr(rc, "adaptive_filtration_s", 0.0))
 
 
 def _ptr(t) -> Optional[int]:
@@ -251,13 +253,15 @@ class Plan:
     """One reconstruction setup (geometry + recon config + execution options)."""
 
     def __init__(self, ctx: Context, scan: S.ScanGeometry, recon: S.ReconConfig,
-                 precision: str = "fp32", tile_z: int = 0, bp_variant: int = 0, stream=None):
+                 precision: str = "fp32", tile_z: int = 0, bp_variant: int = 0, stream=None,
+                 adaptive_filtration: bool = False):
         self.ctx = ctx
         self.scan = scan
         self.recon = recon
         self.precision = precision
         sc, rc = scan_struct(scan), recon_struct(recon)
-        opts = ExecOptionsC(PRECISION[precision], tile_z, bp_variant, 0)
+        opts = ExecOptionsC(PRECISION[precision], tile_z, bp_variant,
+                            1 if adaptive_filtration else 0)
         h = C.c_void_p()
         st = load_library().fct_wfbp_gpu_plan_create(ctx.h, C.byref(sc), C.byref(rc),
                                                      C.byref(opts), _stream_handle(stream),
@@ -386,6 +390,11 @@ class Plan:
                            "reconstruct_host()")
         self.ctx.check(load_library().fct_wfbp_gpu_reconstruct(
             self.h, self._raw(raw), self._volume(volume), 0), "reconstruct")
+
+    def set_output_units(self, units: str = "mu", mu_water: float = S.MU_WATER):
+        """Units of the thickened volume: "mu" (1/mm, default) or "hu"."""
+        self.ctx.check(load_library().fct_wfbp_gpu_set_output_units(
+            self.h, {"mu": 0, "hu": 1}[units], float(mu_water)), "set_output_units")
 
     def _host_volume(self, out):
         shape = (self.info.n_slices_requested, self.recon.ny, self.recon.nx)

@@ -111,8 +111,24 @@ typedef struct {
   int32_t precision;           /* fct_precision */
   int32_t tile_z;              /* slices per backprojection tile: 0 = tuned default, or 8/16/32 */
   int32_t bp_variant;          /* fct_bp_variant */
-  int32_t reserved;
+  /* 1: adaptive filtration of the raw scan with strength
+   * recon.adaptive_filtration_s in (0, 1] before the rebinning (the
+   * reference parses the key and never applies it, parse_config.cpp:57,
+   * README.md:14, so a reference recon file alone does not turn it on):
+   * per view, with vmax the largest |p| of the view, every sample p is
+   * blended with the mean of its 3 x 3 (row, channel) neighbourhood by
+   * lambda = clamp((p - (1 - s) vmax) / (s vmax), 0, 1), smoothing only the
+   * most attenuated (noisiest) rays after the idea of Kachelriess, Watzke and
+   * Kalender, Med. Phys. 28 (2001) 475.  The plan then holds a raw-sized
+   * buffer (not allowed by fct_wfbp_gpu_reconstruct_noalloc). */
+  int32_t adaptive_filtration;
 } fct_exec_options;
+
+/* Units of the thickened volume (fct_wfbp_gpu_set_output_units). */
+typedef enum {
+  FCT_UNITS_MU = 0,            /* linear attenuation, 1/mm (the reference's output) */
+  FCT_UNITS_HU = 1             /* Hounsfield: 1000 (mu - mu_water) / mu_water */
+} fct_units;
 
 typedef struct {
   int32_t n_slices_native;     /* slices at the collimated width (ReconstructionMachine.cu:110-137) */
@@ -190,6 +206,12 @@ FCT_API int fct_wfbp_gpu_backproject(fct_wfbp_gpu_plan* plan, const void* filter
 /* Triangular slice thickening (ReconstructionMachine.cu:387-434,
  * cuda_kernels.cuh:289-320). */
 FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* plan, const float* native, float* volume);
+
+/* Units of the volume the thickening (and every whole-chain entry point)
+ * writes: FCT_UNITS_MU (default, as the reference: no HU conversion
+ * anywhere, ReconstructionMachine.cu:436-455) or FCT_UNITS_HU with
+ * mu_water in 1/mm, fused into the thickening kernel. */
+FCT_API int fct_wfbp_gpu_set_output_units(fct_wfbp_gpu_plan* plan, int32_t units, double mu_water);
 
 /* Whole chain (RunReconstruction, ReconstructionMachine.cu:76-82).  raw and
  * volume are device pointers unless host_buffers != 0, in which case they are

@@ -74,6 +74,7 @@ def lib():
             "or_filter_kernel": (None, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, dp]),
             "or_rebin": (None, [P(OrScan), fp, dp, C.c_int, C.c_int]),
             "or_rebin_entries": (C.c_int, [P(OrScan), i32p, dp]),
+            "or_adaptive_filter": (None, [P(OrScan), C.c_double, fp, fp]),
             "or_filter_rows": (None, [dp, C.c_int64, C.c_int, dp, C.c_int, C.c_int]),
             "or_backproject": (C.c_int, [P(OrScan), P(OrRecon), dp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, dp, i64p]),
@@ -198,6 +199,16 @@ def rebin(scan, raw: np.ndarray, v0=0, v1=None) -> np.ndarray:
     out = np.zeros((v1 - v0, e.n_rows, e.n_channels), np.float64)
     s = scan_struct(scan)
     lib().or_rebin(C.byref(s), _ptr(raw, C.c_float), _ptr(out, C.c_double), v0, v1)
+    return out
+
+
+def adaptive_filter(scan, raw: np.ndarray, strength: float) -> np.ndarray:
+    """Adaptive filtration of a raw scan [view][row][ch] (float32 out)."""
+    raw = np.ascontiguousarray(raw, np.float32)
+    out = np.empty_like(raw)
+    s = scan_struct(scan)
+    lib().or_adaptive_filter(C.byref(s), float(strength), _ptr(raw, C.c_float),
+                             _ptr(out, C.c_float))
     return out
 
 

@@ -352,6 +352,41 @@ void or_rebin(const or_scan* s, const float* raw, double* out, int v0, int v1) {
   }
 }
 
+void or_adaptive_filter(const or_scan* s, double strength, const float* raw, float* out) {
+  // Adaptive filtration of every raw view (new: FreeCT parses
+  // adaptive_filtration_s, parse_config.cpp:57, and applies nothing): vmax
+  // the largest finite |p| of the view; p' = p + lambda (m - p) with m the
+  // mean of the 3 x 3 (row, channel) neighbours that exist and
+  // lambda = clamp((p - (1 - s) vmax) / (s vmax), 0, 1); non-finite -> 0.
+  const int nr = s->n_rows, nc = s->n_channels;
+  const size_t per = (size_t)nr * nc;
+  auto fin = [](float v) { return std::isfinite(v) ? (double)v : 0.0; };
+#pragma omp parallel for schedule(static)
+  for (int v = 0; v < s->n_views; ++v) {
+    const float* x = raw + (size_t)v * per;
+    double vm = 0.0;
+    for (size_t i = 0; i < per; ++i) vm = std::max(vm, std::fabs(fin(x[i])));
+    const double den = strength * vm;
+    for (int r = 0; r < nr; ++r)
+      for (int c = 0; c < nc; ++c) {
+        const double p = fin(x[(size_t)r * nc + c]);
+        const double lam = den > 0.0 ? std::min(std::max((p - (vm - den)) / den, 0.0), 1.0) : 0.0;
+        double res = p;
+        if (lam > 0.0) {
+          double sum = 0.0;
+          int n = 0;
+          for (int rr = std::max(r - 1, 0); rr <= std::min(r + 1, nr - 1); ++rr)
+            for (int cc = std::max(c - 1, 0); cc <= std::min(c + 1, nc - 1); ++cc) {
+              sum += fin(x[(size_t)rr * nc + cc]);
+              ++n;
+            }
+          res = p + lam * (sum / n - p);
+        }
+        out[(size_t)v * per + (size_t)r * nc + c] = (float)res;
+      }
+  }
+}
+
 int or_rebin_entries(const or_scan* s, int32_t* ient, double* fent) {
   // The rebinning table as the plan builds it (fct_ffs::rebin_entry per
   // effective channel and row parity): ient[e*6 + {valid, rho, b0, b1, aoff,

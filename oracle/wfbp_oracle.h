@@ -16,6 +16,8 @@
 //   or_backproject     cuda_kernels.cuh:79-93, 95-196, 198-287
 //   or_thicken         cuda_kernels.cuh:289-320
 //   or_project_phantom new (synthetic scan generator; analytic line integrals)
+//   or_adaptive_filter new (adaptive filtration; the key is parsed at
+//                      parse_config.cpp:57 and never applied)
 //
 // Pins (tests/test_oracle_*.py): the nine reference filter vectors
 // (resources/filters/*.txt), Ram-Lak / Shepp-Logan closed forms, closed-form
@@ -83,6 +85,10 @@ int or_geometry(const or_scan* s, const or_recon* r, or_derived* d);
 int or_slice_grid(double start, double end, double step, double* z, int cap);
 void or_filter_kernel(int n, double ds, int kind, double c, double a, double* h);
 void or_rebin(const or_scan* s, const float* raw, double* out, int v0, int v1);
+// Adaptive filtration of the raw scan with strength s (new; the FreeCT key
+// adaptive_filtration_s is parsed but unused): per view, samples above
+// (1 - s) vmax are blended with their 3 x 3 (row, channel) mean.
+void or_adaptive_filter(const or_scan* s, double strength, const float* raw, float* out);
 // The rebinning table (fct_ffs::rebin_entry, as the plan uploads it):
 // ient[e][valid, rho, b0, b1, aoff, mmax], fent[e][fb, fa]; returns the entry
 // count (call with NULL buffers to size them).

@@ -153,3 +153,54 @@ def test_cli_freect_file_matches_oracle(ctx, oracle, tmp_path, case):
     say("cli freect %s vs fp64: covered %d, rmse %.4f max %.4f HU" % (case, cov.sum(), rmse, mx))
     assert cov.sum() > 0.2 * cov.size
     assert rmse <= 0.5 and mx <= 2.0
+
+
+def _read_nhdr(path):
+    fields, keys = {}, {}
+    lines = open(path).read().splitlines()
+    assert lines[0] == "NRRD0004"
+    for ln in lines[1:]:
+        if not ln or ln.startswith("#"):
+            continue
+        if ":=" in ln:
+            k, v = ln.split(":=", 1)
+            keys[k] = v
+        else:
+            k, v = ln.split(": ", 1)
+            fields[k] = v
+    return fields, keys
+
+
+def test_cli_hounsfield_output_with_sidecar_header(ctx, tmp_path):
+    # output_units: hu and the detached NRRD header written beside the raw
+    # volume: the header's sizes, spacings and data file describe it, and the
+    # data equal the C-ABI volume in HU.
+    w = S.small()
+    rc = w.recon
+    cfg = tmp_path / "recon.yaml"
+    cfg.write_text(f"raw_data_dir: synthetic:small\noutput_dir: {tmp_path}\n"
+                   f"start_pos: {rc.start_pos}\nend_pos: {rc.end_pos}\nrecon_fov: {rc.recon_fov}\n"
+                   f"slice_thickness: {rc.slice_thickness}\nslice_pitch: {rc.slice_pitch}\n"
+                   f"nx: {rc.nx}\nny: {rc.ny}\nfilter: shepp-logan\noutput_units: hu\n"
+                   f"mu_water: {S.MU_WATER}\n")
+    out = tmp_path / "vol.dat"
+    r = subprocess.run([CLI, "-o", str(out), str(cfg)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout
+    fields, keys = _read_nhdr(str(out) + ".nhdr")
+    nx, ny, nz = (int(v) for v in fields["sizes"].split())
+    assert (nx, ny, nz) == (rc.nx, rc.ny, 32)
+    sp = [float(v) for v in fields["spacings"].split()]
+    assert np.allclose(sp, [rc.recon_fov / rc.nx, rc.recon_fov / rc.ny, rc.slice_pitch])
+    assert fields["type"] == "float" and fields["endian"] == "little"
+    assert fields["data file"] == "vol.dat" and keys["freect_units"] == "HU"
+    got = np.fromfile(tmp_path / fields["data file"], np.float32).reshape(nz, ny, nx)
+    from freect_amd import wfbp
+    raw = ctx.phantom_project(w.scan, w.phantom, w.i0, w.seed)
+    p = wfbp.Plan(ctx, w.scan, w.recon)
+    p.set_output_units("hu", S.MU_WATER)
+    vol = p.alloc_volume()
+    p.reconstruct(raw, vol)
+    p.synchronize()
+    say("cli HU output: max |diff| %.3g HU" % np.abs(got - to_np(vol)).max())
+    assert np.array_equal(got, to_np(vol))

@@ -119,3 +119,19 @@ def test_freect_format_truncated_file_is_an_error(tmp_path, oracle):
     open(tmp_path / "projections.dat", "wb").write(data[: len(data) // 2])
     rc, out = run("--inspect", str(tmp_path))
     assert rc != 0 and "shorter" in out
+
+
+def test_parse_config_engine_keys(tmp_path):
+    cfg = tmp_path / "engine.yaml"
+    cfg.write_text("start_pos: 0\nend_pos: 10\nadaptive_filtration_s: 0.25\n"
+                   "adaptive_filtration: 1\noutput_units: hu\nmu_water: 0.0195\n")
+    rc, out = run("--print-config", str(cfg))
+    assert rc == 0
+    p = json.loads(out)
+    assert p["adaptive_filtration"] == 1 and abs(p["adaptive_filtration_s"] - 0.25) < 1e-7
+    assert p["output_units"] == "hu" and abs(p["mu_water"] - 0.0195) < 1e-12
+    # A reference recon file (adaptive_filtration_s: 0.2, RawDataSet.cpp:77) alone
+    # leaves the filtration off and the output in 1/mm, as the reference does.
+    cfg.write_text("start_pos: 0\nend_pos: 10\nadaptive_filtration_s: 0.2\n")
+    p = json.loads(run("--print-config", str(cfg))[1])
+    assert p["adaptive_filtration"] == 0 and p["output_units"] == "mu"
