@@ -56,7 +56,7 @@ def _compile(src: str) -> str:
 
 # Host C++ above the C ABI (mirror of FreeCT's ReconstructionMachine,
 # parse_config and RawDataSet), compiled by the host compiler.
-HOST_SOURCES = ["host/fct_host.cpp"]
+HOST_SOURCES = ["host/fct_host.cpp", "host/dicom.cpp"]
 CLI_SOURCE = "host/main.cpp"
 CLI_PATH = os.path.join(LIBDIR, "fct_wfbp")
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-ffp-contract=off",

@@ -640,12 +640,17 @@ int ReconstructionMachine::LoadReconConfiguration(const std::string& filepath) {
   return parse_config(filepath, m_rp, m_verbose, &m_err);
 }
 
+// raw_data_dir: "synthetic:<workload>", a FreeCT two-file directory
+// (meta.yaml + projections.dat), or a directory of DICOM-CT-PD files.
+std::shared_ptr<RawDataSet> open_data_set(const std::string& dir, int device) {
+  if (dir.rfind("synthetic:", 0) == 0) return std::make_shared<SyntheticDataSet>(device);
+  if (file_exists(dir + "/meta.yaml")) return std::make_shared<FreeCTDataSet>();
+  return std::make_shared<DicomDataSet>();
+}
+
 int ReconstructionMachine::LoadRawData() {
   if (m_verbose) std::cout << "ReconstructionMachine: Loading raw data..." << std::endl;
-  if (m_rp.raw_data_dir.rfind("synthetic:", 0) == 0)
-    m_org_data_set = std::make_shared<SyntheticDataSet>(m_device);
-  else
-    m_org_data_set = std::make_shared<FreeCTDataSet>();
+  m_org_data_set = open_data_set(m_rp.raw_data_dir, m_device);
   m_org_data_set->setPath(m_rp.raw_data_dir);
   int st = m_org_data_set->initialize();
   if (!st) {

@@ -169,6 +169,20 @@ class FreeCTDataSet : public RawDataSet {
   int64_t m_next_frame = -1;   // frame the open stream is positioned at
 };
 
+// DICOM-CT-PD projection files (one projection per file, sorted by name),
+// read by a self-written DICOM parser (dicom.cpp) in place of DCMTK:
+// DicomDataSet.cpp:56-330 and its private-tag map (:178-211).
+class DicomDataSet : public RawDataSet {
+ public:
+  int initialize() override;   // file list, then the scan metadata of the first file
+  int readMetadata() override;
+  int readProjection(int projection_idx) override;
+  int readAll() override;      // every file, on the host's threads
+
+ private:
+  std::vector<std::string> m_file_list;
+};
+
 // Seeded synthetic helical scan (water cylinder + ellipsoids, analytic line
 // integrals on the GPU, optional Poisson noise).  raw_data_dir
 // "synthetic:<workload>[:seed=<n>]" with workload small | full | adjoint.
@@ -203,6 +217,10 @@ bool fct_synthetic_workload(const std::string& name, uint64_t seed, fct_scan_geo
                             uint64_t* seed_used);
 void fct_make_phantom(uint64_t seed, double z_lo, double z_hi, int n,
                       std::vector<fct_ellipsoid>* out);
+
+// The data set a raw_data_dir names: "synthetic:<workload>", a FreeCT
+// two-file directory (meta.yaml present) or a DICOM-CT-PD directory.
+std::shared_ptr<RawDataSet> open_data_set(const std::string& raw_data_dir, int device = 0);
 
 class ReconstructionMachine {
  public:

@@ -4,7 +4,7 @@
 //   fct_wfbp --write-synthetic <small|full|adjoint> <dir> [--seed n]
 //   fct_wfbp --print-phantom <small|full|adjoint> [--seed n]
 //   fct_wfbp --print-config recon.yaml      (parsed recon keys, JSON)
-//   fct_wfbp --inspect <freect dir>         (reader + derived scan law, JSON)
+//   fct_wfbp --inspect <freect or DICOM-CT-PD dir>  (reader + derived scan law, JSON)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -111,7 +111,7 @@ int main(int argc, char** argv) {
   }
 
   if (!inspect.empty()) {
-    auto ds = std::make_shared<fct::FreeCTDataSet>();
+    auto ds = fct::open_data_set(inspect, device);
     ds->setPath(inspect);
     if (ds->initialize() || ds->readAll()) return fail("FreeCTDataSet", ds->m_last_error);
     double sum = 0.0;
@@ -129,11 +129,13 @@ int main(int argc, char** argv) {
                 "\"angle0\": %.17g, \"z0\": %.17g, \"feed\": %.17g, \"dsd\": %.17g, \"r\": %.17g, "
                 "\"channel_spacing\": %.17g, \"row_spacing\": %.17g, \"central_channel\": %.17g, "
                 "\"central_row\": %.17g, \"sum\": %.17g, \"angle_last\": %.9g, \"z_last\": %.9g, "
-                "\"rotation\": %d, \"ffs_mode\": %d, \"anode_angle\": %.17g}\n",
+                "\"rotation\": %d, \"ffs_mode\": %d, \"anode_angle\": %.17g, "
+                "\"manufacturer\": \"%s\", \"detector_shape\": \"%s\"}\n",
                 g.n_views, g.n_rows, g.n_channels, g.views_per_rotation, g.angle0, g.z0, g.feed,
                 g.dsd, g.r, g.channel_spacing, g.row_spacing, g.central_channel, g.central_row,
                 sum, ds->getTubeAngle(g.n_views - 1), ds->getTablePosition(g.n_views - 1),
-                g.rotation, g.ffs_mode, g.anode_angle);
+                g.rotation, g.ffs_mode, g.anode_angle, ds->getManufacturer().c_str(),
+                ds->getDetectorShape().c_str());
     return 0;
   }
 

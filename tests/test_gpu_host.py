@@ -204,3 +204,58 @@ def test_cli_hounsfield_output_with_sidecar_header(ctx, tmp_path):
     p.synchronize()
     say("cli HU output: max |diff| %.3g HU" % np.abs(got - to_np(vol)).max())
     assert np.array_equal(got, to_np(vol))
+
+
+def test_cli_dicom_ct_pd_matches_freect_format(ctx, oracle, tmp_path):
+    # The same quantized scan as DICOM-CT-PD files (implicit VR, transposed
+    # pixel data) and as a FreeCT two-file directory holding the values the
+    # DICOM rescale produces: the driver reconstructs both to the same volume,
+    # and that volume matches the FP64 restatement on the covered voxels.
+    import dicom_writer as DW
+    w = S.tiny(nx=48, n_slices=12, n_channels=192, n_rows=16, ppr=192, rotations=3)
+    sc, rc = w.scan, w.recon
+    raw = oracle.project_phantom(sc, w.phantom)
+    v = np.arange(sc.n_views)
+    angles = sc.angle0 + v * 2 * np.pi / sc.views_per_rotation
+    zs = sc.z0 + v * sc.feed / sc.views_per_rotation
+    ddir, fdir = tmp_path / "dicom", tmp_path / "freect"
+    ddir.mkdir()
+    fdir.mkdir()
+    deq = DW.write_scan(str(ddir), sc, raw, angles, zs, syntax=DW.IMPLICIT, transposed=True)
+    _write_freect(str(fdir), sc, deq, angles, zs)
+    vols = {}
+    for name, d in (("dicom", ddir), ("freect", fdir)):
+        cfg = tmp_path / f"{name}.yaml"
+        cfg.write_text(f"raw_data_dir: {d}\noutput_dir: {tmp_path}\nstart_pos: {rc.start_pos!r}\n"
+                       f"end_pos: {rc.end_pos!r}\nrecon_fov: {rc.recon_fov!r}\n"
+                       f"slice_thickness: {rc.slice_thickness!r}\n"
+                       f"slice_pitch: {rc.slice_pitch!r}\nnx: {rc.nx}\nny: {rc.ny}\n"
+                       f"filter: {rc.filter}\n")
+        out = tmp_path / f"{name}.dat"
+        say("cli", name)
+        r = subprocess.run([CLI, "-o", str(out), str(cfg)], capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout
+        vols[name] = np.fromfile(out, np.float32)
+    assert np.array_equal(vols["dicom"], vols["freect"])
+    import json
+    g = json.loads(subprocess.run([CLI, "--inspect", str(ddir)], capture_output=True, text=True,
+                                  timeout=300).stdout)
+    osc = sc.replace(channel_spacing=g["channel_spacing"], row_spacing=g["row_spacing"],
+                     dsd=g["dsd"], r=g["r"], central_channel=g["central_channel"],
+                     central_row=g["central_row"], angle0=g["angle0"], z0=g["z0"],
+                     feed=g["feed"])
+    f32 = lambda x: float(np.float32(x))   # noqa: E731
+    orc = rc.replace(start_pos=f32(rc.start_pos), end_pos=f32(rc.end_pos),
+                     recon_fov=f32(rc.recon_fov), slice_thickness=f32(rc.slice_thickness),
+                     slice_pitch=f32(rc.slice_pitch))
+    d = oracle.geometry(osc, orc)
+    nat, _ = oracle.reconstruct_native(osc, orc, deq, 0, d.n_slices_native)
+    ref = oracle.thicken(osc, orc, nat)
+    from helpers import fov_mask
+    cov = oracle.covered_mask(osc, orc) & fov_mask(orc, d.acq_fov)[None]
+    got = vols["dicom"].reshape(ref.shape)
+    dd = (got.astype(np.float64) - ref)[cov] / (S.MU_WATER / 1000.0)
+    rmse, mx = float(np.sqrt(np.mean(dd * dd))), float(np.abs(dd).max())
+    say("cli dicom vs fp64: rmse %.4f max %.4f HU" % (rmse, mx))
+    assert rmse <= 0.5 and mx <= 2.0
