@@ -9,12 +9,18 @@ P0  : R^T M^T, a matched helical projector: M^T projects the volume onto the
       mode), R^T is the rebinning transpose; P0^T = M R (rebinning, then the
       R / l-weighted plain sum of backproject_v1.hip's matched mode).
 
+Pr  : the ray-driven projector (Joseph's method, rayproj.hip): line
+      integrals of the native volume along the raw rays; Pr^T its exact
+      transpose.
+
 WFBPFunction.apply(p) = A p with backward g -> A^T g; ProjectFunction.apply(x)
 = A^T x with backward g -> A g; MatchedProjectFunction.apply(x) = P0 x with
-backward g -> P0^T g.  `refine` runs gradient steps on 0.5 ||k P x - p||^2
-from the WFBP volume, P = P0 (default; k fitted by least squares at the start
-so that k P0 x0 matches p) or P = A^T (k = 1, the objective whose minimiser is
-not the attenuation map, since A^T carries the ramp filter), with step
+backward g -> P0^T g; RayProjectFunction.apply(x) = Pr x with backward
+g -> Pr^T g.  `refine` runs gradient steps on 0.5 ||k P x - p||^2 from the
+WFBP volume, P = Pr (default, k = 1: a model of the measured line
+integrals), P = P0 (k fitted by least squares at the start so that k P0 x0
+matches p) or P = A^T (k = 1, the objective whose minimiser is not the
+attenuation map, since A^T carries the ramp filter), with step
 1 / (k^2 ||P P^T||) from power iteration.
 """
 from __future__ import annotations
@@ -89,6 +95,36 @@ def matched_project(x, plan):
     return MatchedProjectFunction.apply(x, plan)
 
 
+def _pr(plan, x):
+    p = plan.alloc_raw()
+    plan.ray_project(x.contiguous(), p)
+    return p
+
+
+def _prt(plan, g):
+    x = plan.alloc_native()
+    plan.ray_backproject(g.contiguous(), x)
+    return x
+
+
+class RayProjectFunction(torch.autograd.Function):
+    """Pr x: line integrals of the native volume along the raw rays (Joseph's
+    method, rayproj.hip); backward g -> Pr^T g, its exact transpose."""
+
+    @staticmethod
+    def forward(ctx, x, plan):
+        ctx.plan = plan
+        return _pr(plan, x)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _prt(ctx.plan, g), None
+
+
+def ray_project(x, plan):
+    return RayProjectFunction.apply(x, plan)
+
+
 def wfbp(raw, plan):
     return WFBPFunction.apply(raw, plan)
 
@@ -109,6 +145,8 @@ def power_iteration(plan, shape, iters: int = 12, seed: int = 0,
         with torch.no_grad():
             if operator == "matched":
                 w = _p0t(plan, _p0(plan, v))
+            elif operator == "ray":
+                w = _prt(plan, _pr(plan, v))
             else:
                 p = plan.alloc_raw()
                 plan.adjoint(v, p)
@@ -144,9 +182,10 @@ def inside_rays(plan, margin: float = 1.0):
 
 
 def refine(plan, raw, x0, steps: int = 10, mode: str = "store", lam: float = None,
-           callback=None, operator: str = "matched", mask=None):
+           callback=None, operator: str = "ray", mask=None):
     """Gradient descent on 0.5 ||k P x - p||^2 from x0 with step 1 / (k^2 lam).
 
+    operator "ray": P = Pr, the ray-driven line-integral projector, k = 1;
     operator "matched": P = P0 = R^T M^T, k = <P0 x0, p> / <P0 x0, P0 x0>;
     operator "adjoint": P = A^T, k = 1.
     mask (broadcastable to the raw data, e.g. inside_rays(plan)) restricts
@@ -158,7 +197,7 @@ def refine(plan, raw, x0, steps: int = 10, mode: str = "store", lam: float = Non
     """
     from torch.utils.checkpoint import checkpoint
 
-    proj = matched_project if operator == "matched" else project
+    proj = {"matched": matched_project, "ray": ray_project, "adjoint": project}[operator]
     m = mask if mask is not None else torch.ones((), device=raw.device)
     with torch.no_grad():
         if operator == "matched":

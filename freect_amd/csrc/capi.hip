@@ -251,6 +251,52 @@ namespace {
 
 fct::BpArgs bp_args(const fct_wfbp_gpu_plan* p) { return fct::plan_bp_args(p); }
 
+// The ray-driven projector's geometry: the raw scan's rays and the native
+// voxel grid in the plan's frame (a clockwise scan as its mirror image: tube
+// angles -a_v counter-clockwise, y -> -y, like the voxel tables).
+fct::RayProjArgs ray_args(const fct_wfbp_gpu_plan* p) {
+  const fct_scan_geometry& s = p->raw_scan;
+  const fct_recon_config& r = p->recon;
+  fct::RayProjArgs a{};
+  a.ncols = s.n_channels;
+  a.nrows = s.n_rows;
+  a.v0 = 0;
+  a.v1 = s.n_views;
+  a.nx = r.nx;
+  a.ny = r.ny;
+  a.nz = p->n_native;
+  a.r = s.r;
+  a.dsd = s.dsd;
+  a.dgamma = s.channel_spacing / s.dsd;
+  a.drow = s.row_spacing;
+  a.cc = s.central_channel;
+  a.cr = s.central_row;
+  const bool cw = s.rotation < 0;
+  a.angle0 = cw ? -s.angle0 : s.angle0;
+  a.dtheta = 2.0 * kPi / (double)s.views_per_rotation;
+  a.z0 = s.z0;
+  a.dz = s.feed / (double)s.views_per_rotation;
+  a.states = p->ffs.S;
+  for (int rho = 0; rho < p->ffs.S; ++rho) {
+    const fct_ffs::Deflection d = fct_ffs::state_offset(p->ffs.mode, rho, p->ffs.dr, p->ffs.da,
+                                                        p->ffs.anode);
+    a.ffs_off[rho][0] = d.dr;
+    a.ffs_off[rho][1] = d.da;
+    a.ffs_off[rho][2] = d.dz;
+  }
+  const double dx = r.recon_fov / r.nx, dy = r.recon_fov / r.ny;
+  const double x0 = -0.5 * (r.nx - 1) * dx + r.x_origin, y0 = -0.5 * (r.ny - 1) * dy + r.y_origin;
+  a.gx0 = (float)x0;
+  a.gdx = (float)dx;
+  a.gy0 = (float)(cw ? -y0 : y0);
+  a.gdy = (float)(cw ? -dy : dy);
+  const double z0 = fct_tables::slice_z(r.start_pos, r.end_pos, p->csw, 0);
+  const double z1 = fct_tables::slice_z(r.start_pos, r.end_pos, p->csw, 1);
+  a.gz0 = (float)z0;
+  a.gdz = (float)(z1 - z0);
+  return a;
+}
+
 // Rebin + filter of every line of the scan (line0 / nlines narrow it).
 fct::RebinFilterArgs rf_args(const fct_wfbp_gpu_plan* p, const float* raw, void* out,
                              float* scale) {
@@ -849,6 +895,34 @@ FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* p, const float* native, floa
   a.hu = p->out_hu;
   a.hu_scale = p->out_hu ? (float)(1000.0 / p->mu_water) : 1.0f;
   HIP_TRY(p->ctx, fct::launch_thicken(a, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_ray_project(fct_wfbp_gpu_plan* p, const float* native, float* raw) {
+  if (!p || !native || !raw) return FCT_ERR_INVALID_ARGUMENT;
+  if (p->n_native < 2)
+    return set_err(p->ctx, "the ray projector needs at least two native slices"),
+           FCT_ERR_UNSUPPORTED;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  fct::RayProjArgs a = ray_args(p);
+  a.vol_in = native;
+  a.raw_out = raw;
+  HIP_TRY(p->ctx, fct::launch_ray_project(a, false, p->stream));
+  return FCT_OK;
+}
+
+FCT_API int fct_wfbp_gpu_ray_backproject(fct_wfbp_gpu_plan* p, const float* raw, float* native) {
+  if (!p || !native || !raw) return FCT_ERR_INVALID_ARGUMENT;
+  if (p->n_native < 2)
+    return set_err(p->ctx, "the ray projector needs at least two native slices"),
+           FCT_ERR_UNSUPPORTED;
+  HIP_TRY(p->ctx, hipSetDevice(p->device));
+  fct::RayProjArgs a = ray_args(p);
+  a.raw_in = raw;
+  a.vol_out = native;
+  HIP_TRY(p->ctx, fct::launch_fill(native, (size_t)p->n_native * p->recon.ny * p->recon.nx, 0.0f,
+                                   p->stream));
+  HIP_TRY(p->ctx, fct::launch_ray_project(a, true, p->stream));
   return FCT_OK;
 }
 

@@ -409,9 +409,30 @@ void rebin_filter_reg_kernel(RebinFilterArgs a) {
     if (blockIdx.x == 0 && t == 0) a.scale[0] = s;
   }
 
+  // Order of the line pairs.  With an even row count a pair is rows
+  // (2i, 2i + 1) of one view, and the pairs are taken row pair by row pair,
+  // sweeping the views: the blocks running together then read the same two
+  // fan rows of neighbouring views, which stay in L2 (a parallel view reads
+  // the fan views within about half a fan of its own).  The list is split
+  // into eight contiguous ranges, one per XCD (block b runs on XCD b mod 8).
+  // Odd row counts keep the line order (pairs straddle views).
   const int npairs = (a.nlines + 1) >> 1;
-  for (int pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
-    const int l0 = a.line0 + 2 * pair, l1 = l0 + 1;
+  const bool by_row = (a.nrows & 1) == 0 && (gridDim.x & 7) == 0;
+  const int nv_range = a.nlines / a.nrows, j_first = a.line0 / a.nrows;
+  const int xcd = blockIdx.x & 7, nb8 = gridDim.x >> 3;
+  const int p_lo = by_row ? (int)(((long long)npairs * xcd) >> 3) : 0;
+  const int p_hi = by_row ? (int)(((long long)npairs * (xcd + 1)) >> 3) : npairs;
+  const int p_step = by_row ? nb8 : (int)gridDim.x;
+  for (int pair = p_lo + (by_row ? (int)(blockIdx.x >> 3) : (int)blockIdx.x); pair < p_hi;
+       pair += p_step) {
+    int l0;
+    if (by_row) {
+      const int rp = pair / nv_range;
+      l0 = (j_first + (pair - rp * nv_range)) * a.nrows + 2 * rp;
+    } else {
+      l0 = a.line0 + 2 * pair;
+    }
+    const int l1 = l0 + 1;
     const bool has1 = l1 < a.line0 + a.nlines;
     const int j0 = l0 / a.nrows, r0 = l0 - j0 * a.nrows;
     const int j1 = has1 ? l1 / a.nrows : 0, r1 = has1 ? l1 - j1 * a.nrows : 0;
@@ -552,7 +573,8 @@ hipError_t launch_reg(const RebinFilterArgs& a, hipStream_t s) {
     per_cu = nb;
   }
   const int npairs = (a.nlines + 1) / 2;
-  const int blocks = std::max(1, std::min(npairs, per_cu * std::max(a.ncu, 1)));
+  int blocks = std::max(1, std::min(npairs, per_cu * std::max(a.ncu, 1)));
+  if (blocks >= 8) blocks &= ~7;   // whole XCD groups (the row-pair-major order)
   hipLaunchKernelGGL(rebin_filter_reg_kernel<N>, dim3(blocks), dim3(N / 8), 0, s, a);
   return hipGetLastError();
 }

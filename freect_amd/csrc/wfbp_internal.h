@@ -133,6 +133,19 @@ struct PhantomArgs {
   double ffs_off[4][3];      // per state (dr, da, dz) along (e_w, e_u, e_z)
 };
 
+// Ray-driven projector of the native volume onto the raw fan data
+// (rayproj.hip), in the plan's frame (a clockwise scan mirrored).
+struct RayProjArgs {
+  const float* vol_in;       // project: native [slice][y][x]
+  float* vol_out;            // transpose: native, accumulated into
+  const float* raw_in;       // transpose: [view - v0][row][ch]
+  float* raw_out;            // project: [view - v0][row][ch]
+  int32_t ncols, nrows, v0, v1, nx, ny, nz, states;
+  double r, dsd, dgamma, drow, cc, cr, angle0, dtheta, z0, dz;
+  double ffs_off[4][3];      // per focal state (dr, da, dz) along (e_w, e_u, e_z)
+  float gx0, gdx, gy0, gdy, gz0, gdz;   // voxel-centre grids (FP32 of FP64)
+};
+
 struct BpAdjArgs {
   BpArgs bp;                 // geometry tables (filt/native unused)
   const float* xbar;         // [slice][y][x] native-volume cotangent
@@ -148,6 +161,9 @@ struct RebinAdjArgs {
 };
 
 hipError_t launch_rebin_filter(const RebinFilterArgs& a, hipStream_t s);
+// Ray-driven projection of vol_in onto raw_out (views [v0, v1)), or with
+// transpose its exact transpose from raw_in into vol_out (accumulated).
+hipError_t launch_ray_project(const RayProjArgs& a, bool transpose, hipStream_t s);
 // Adaptive filtration of raw views [v0, v1): per view the largest finite
 // |p| (vmax), then every sample blended with its 3 x 3 (row, channel) mean by
 // lambda = clamp((p - (1 - s) vmax) / (s vmax), 0, 1).  out != in.

@@ -89,6 +89,8 @@ SIGNATURES = {
     "fct_wfbp_gpu_reconstruct": (C.c_int, [_VP, _VP, _VP, C.c_int32]),
     "fct_wfbp_gpu_reconstruct_stream": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_float, _VP]),
     "fct_wfbp_gpu_set_output_units": (C.c_int, [_VP, C.c_int32, C.c_double]),
+    "fct_wfbp_gpu_ray_project": (C.c_int, [_VP, _VP, _VP]),
+    "fct_wfbp_gpu_ray_backproject": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_backproject_adjoint": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_filter": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_rebin_adjoint": (C.c_int, [_VP, _VP, _VP]),
@@ -484,6 +486,16 @@ class Plan:
         """Rebinning without the filter: FP32 [view][row][channel_pitch]."""
         self.ctx.check(load_library().fct_wfbp_gpu_rebin(
             self.h, self._raw(raw), self._lines(lines_out, "lines_out")), "rebin")
+
+    def ray_project(self, native, raw):
+        """Line integrals of the native volume along every raw ray (Joseph)."""
+        self.ctx.check(load_library().fct_wfbp_gpu_ray_project(
+            self.h, self._native(native), self._raw(raw)), "ray_project")
+
+    def ray_backproject(self, raw, native):
+        """Exact transpose of ray_project (native overwritten)."""
+        self.ctx.check(load_library().fct_wfbp_gpu_ray_backproject(
+            self.h, self._raw(raw), self._native(native)), "ray_backproject")
 
     def matched_project(self, native, lines):
         self.ctx.check(load_library().fct_wfbp_gpu_matched_project(

@@ -283,6 +283,17 @@ FCT_API int fct_wfbp_gpu_matched_backproject(fct_wfbp_gpu_plan* plan, const floa
                                              float* native);
 FCT_API int fct_wfbp_gpu_rebin_adjoint(fct_wfbp_gpu_plan* plan, const float* filtered_bar,
                                        float* raw_bar);
+/* Ray-driven projector (no reference counterpart; the forward model of the
+ * data-consistency refinement): line integrals of the native volume
+ * [slice][y][x] along every raw ray (the focal spot of projection v, with
+ * its FFS deflection, to its detector cell) by Joseph's method -- at each
+ * voxel plane of the in-plane axis the ray is most aligned with, the volume
+ * interpolated bilinearly in the other in-plane axis and z, zero outside --
+ * into raw [view][row][ch] (overwritten); and its exact transpose from raw
+ * rays into the native volume (overwritten). */
+FCT_API int fct_wfbp_gpu_ray_project(fct_wfbp_gpu_plan* plan, const float* native, float* raw);
+FCT_API int fct_wfbp_gpu_ray_backproject(fct_wfbp_gpu_plan* plan, const float* raw,
+                                         float* native);
 FCT_API int fct_wfbp_gpu_thicken_adjoint(fct_wfbp_gpu_plan* plan, const float* volume_bar,
                                          float* native_bar);
 FCT_API int fct_wfbp_gpu_adjoint(fct_wfbp_gpu_plan* plan, const float* native_bar, float* raw_bar);

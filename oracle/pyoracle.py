@@ -75,6 +75,8 @@ def lib():
             "or_rebin": (None, [P(OrScan), fp, dp, C.c_int, C.c_int]),
             "or_rebin_entries": (C.c_int, [P(OrScan), i32p, dp]),
             "or_adaptive_filter": (None, [P(OrScan), C.c_double, fp, fp]),
+            "or_ray_project": (C.c_int, [P(OrScan), P(OrRecon), dp, dp]),
+            "or_ray_backproject": (C.c_int, [P(OrScan), P(OrRecon), dp, dp]),
             "or_filter_rows": (None, [dp, C.c_int64, C.c_int, dp, C.c_int, C.c_int]),
             "or_backproject": (C.c_int, [P(OrScan), P(OrRecon), dp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, dp, i64p]),
@@ -210,6 +212,27 @@ def adaptive_filter(scan, raw: np.ndarray, strength: float) -> np.ndarray:
     lib().or_adaptive_filter(C.byref(s), float(strength), _ptr(raw, C.c_float),
                              _ptr(out, C.c_float))
     return out
+
+
+def ray_project(scan, rc, vol: np.ndarray) -> np.ndarray:
+    """Joseph line integrals of a native volume along the raw rays (FP64)."""
+    vol = np.ascontiguousarray(vol, np.float64)
+    raw = np.zeros((scan.n_views, scan.n_rows, scan.n_channels), np.float64)
+    s, r = scan_struct(scan), recon_struct(rc)
+    if lib().or_ray_project(C.byref(s), C.byref(r), _ptr(vol, C.c_double), _ptr(raw, C.c_double)):
+        raise RuntimeError("or_ray_project failed")
+    return raw
+
+
+def ray_backproject(scan, rc, raw: np.ndarray, n_slices: int) -> np.ndarray:
+    """Exact transpose of ray_project."""
+    raw = np.ascontiguousarray(raw, np.float64)
+    vol = np.zeros((n_slices, rc.ny, rc.nx), np.float64)
+    s, r = scan_struct(scan), recon_struct(rc)
+    if lib().or_ray_backproject(C.byref(s), C.byref(r), _ptr(raw, C.c_double),
+                                _ptr(vol, C.c_double)):
+        raise RuntimeError("or_ray_backproject failed")
+    return vol
 
 
 def rebin_entries(scan):

@@ -352,6 +352,118 @@ void or_rebin(const or_scan* s, const float* raw, double* out, int v0, int v1) {
   }
 }
 
+}  // extern "C"
+
+namespace {
+
+// Joseph's method (P. M. Joseph, IEEE TMI 1 (1982) 192) along one ray
+// s + t u of the physical scan (clockwise rays traced directly): at every
+// voxel plane of the in-plane axis u is most aligned with, bilinear in the
+// other in-plane axis and z, zero outside, times the path length per plane.
+// f(index, weight) is called for every voxel a plane touches.
+template <class F>
+double joseph(const or_recon* r, int nz, double gz0, double gdz, const double* s, const double* u,
+              F&& f) {
+  const double dx = r->recon_fov / r->nx, dy = r->recon_fov / r->ny;
+  const double x0 = -0.5 * (r->nx - 1) * dx + r->x_origin, y0 = -0.5 * (r->ny - 1) * dy + r->y_origin;
+  const bool xm = std::fabs(u[0]) >= std::fabs(u[1]);
+  const double p0 = xm ? x0 : y0, dp = xm ? dx : dy, q0 = xm ? y0 : x0, dq = xm ? dy : dx;
+  const double sp = xm ? s[0] : s[1], sq = xm ? s[1] : s[0];
+  const double up = xm ? u[0] : u[1], uq = xm ? u[1] : u[0];
+  const int np = xm ? r->nx : r->ny, nq = xm ? r->ny : r->nx;
+  for (int i = 0; i < np; ++i) {
+    const double t = (p0 + i * dp - sp) / up;
+    const double fq = (sq + t * uq - q0) / dq, fz = (s[2] + t * u[2] - gz0) / gdz;
+    if (!(fq > -1.0 && fq < nq) || !(fz > -1.0 && fz < nz)) continue;
+    const double jq = std::floor(fq), kz = std::floor(fz);
+    const double wq = fq - jq, wz = fz - kz;
+    for (int dk = 0; dk < 2; ++dk)
+      for (int dj = 0; dj < 2; ++dj) {
+        const int jj = (int)jq + dj, kk = (int)kz + dk;
+        if (jj < 0 || jj >= nq || kk < 0 || kk >= nz) continue;
+        const int ix = xm ? i : jj, iy = xm ? jj : i;
+        f(((size_t)kk * r->ny + iy) * r->nx + ix, (dj ? wq : 1 - wq) * (dk ? wz : 1 - wz));
+      }
+  }
+  return std::fabs(dp / up);
+}
+
+// Ray of raw projection v, row, channel (the phantom projector's rays).
+void scan_ray(const or_scan* s, const fct_ffs::Geometry& F, int v, int row, int c, double* src,
+              double* u) {
+  const double sg = s->rotation < 0 ? -1.0 : 1.0;
+  const double dth = 2.0 * kPi / s->views_per_rotation, dz = s->feed / s->views_per_rotation;
+  const double a = s->angle0 + sg * dth * v;
+  const fct_ffs::Deflection fd = fct_ffs::state_offset(F.mode, v % F.S, F.dr, F.da, F.anode);
+  const double ox = fd.dr * std::cos(a) - sg * fd.da * std::sin(a);
+  const double oy = fd.dr * std::sin(a) + sg * fd.da * std::cos(a);
+  src[0] = s->r * std::cos(a) + ox;
+  src[1] = s->r * std::sin(a) + oy;
+  src[2] = s->z0 + dz * v + fd.dz;
+  const double beta = sg * ((double)c - s->central_channel) * (s->channel_spacing / s->dsd);
+  const double d[3] = {-s->dsd * std::cos(a + beta) - ox, -s->dsd * std::sin(a + beta) - oy,
+                       ((double)row - s->central_row) * s->row_spacing - fd.dz};
+  const double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  for (int k = 0; k < 3; ++k) u[k] = d[k] / len;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Line integrals of the native volume along the raw rays by Joseph's
+// method (transpose = 0: vol -> raw, raw overwritten), or its transpose
+// (transpose = 1: raw -> vol, vol overwritten).
+static int ray_core(const or_scan* s, const or_recon* r, double* vol, double* raw, int transpose) {
+  or_derived d;
+  const or_scan es = effective(s);
+  if (or_geometry(&es, r, &d) != 0 || d.n_slices_native < 2) return -1;
+  const int nz = d.n_slices_native;
+  const double gz0 = fct_tables::slice_z(r->start_pos, r->end_pos, d.csw, 0);
+  const double gdz = fct_tables::slice_z(r->start_pos, r->end_pos, d.csw, 1) - gz0;
+  const fct_ffs::Geometry F = ffs_geometry(s);
+  const size_t per = (size_t)s->n_rows * s->n_channels;
+  const size_t nvol = (size_t)nz * r->ny * r->nx;
+  if (transpose) std::fill(vol, vol + nvol, 0.0);
+#pragma omp parallel
+  {
+    std::vector<double> acc(transpose ? nvol : 0, 0.0);
+#pragma omp for schedule(static)
+    for (int v = 0; v < s->n_views; ++v)
+      for (int row = 0; row < s->n_rows; ++row)
+        for (int c = 0; c < s->n_channels; ++c) {
+          double src[3], u[3];
+          scan_ray(s, F, v, row, c, src, u);
+          const size_t o = (size_t)v * per + (size_t)row * s->n_channels + c;
+          if (!transpose) {
+            double sum = 0.0;
+            const double len =
+                joseph(r, nz, gz0, gdz, src, u, [&](size_t i, double w) { sum += w * vol[i]; });
+            raw[o] = sum * len;
+          } else if (raw[o] != 0.0) {
+            const bool xm = std::fabs(u[0]) >= std::fabs(u[1]);
+            const double len =
+                std::fabs((xm ? r->recon_fov / r->nx : r->recon_fov / r->ny) / (xm ? u[0] : u[1]));
+            const double sv = raw[o] * len;
+            joseph(r, nz, gz0, gdz, src, u, [&](size_t i, double w) { acc[i] += w * sv; });
+          }
+        }
+    if (transpose) {
+#pragma omp critical
+      for (size_t i = 0; i < nvol; ++i) vol[i] += acc[i];
+    }
+  }
+  return 0;
+}
+
+int or_ray_project(const or_scan* s, const or_recon* r, const double* vol, double* raw) {
+  return ray_core(s, r, const_cast<double*>(vol), raw, 0);
+}
+
+int or_ray_backproject(const or_scan* s, const or_recon* r, const double* raw, double* vol) {
+  return ray_core(s, r, vol, const_cast<double*>(raw), 1);
+}
+
 void or_adaptive_filter(const or_scan* s, double strength, const float* raw, float* out) {
   // Adaptive filtration of every raw view (new: FreeCT parses
   // adaptive_filtration_s, parse_config.cpp:57, and applies nothing): vmax

@@ -85,6 +85,13 @@ int or_geometry(const or_scan* s, const or_recon* r, or_derived* d);
 int or_slice_grid(double start, double end, double step, double* z, int cap);
 void or_filter_kernel(int n, double ds, int kind, double c, double a, double* h);
 void or_rebin(const or_scan* s, const float* raw, double* out, int v0, int v1);
+// Ray-driven projector (new; the refinement's line-integral model): Joseph's
+// method along every raw ray of the physical scan (FFS and clockwise rays
+// traced directly) through the native volume [slice][y][x], zero outside;
+// raw [view][row][ch] overwritten.  or_ray_backproject: its exact transpose
+// (vol overwritten).
+int or_ray_project(const or_scan* s, const or_recon* r, const double* vol, double* raw);
+int or_ray_backproject(const or_scan* s, const or_recon* r, const double* raw, double* vol);
 // Adaptive filtration of the raw scan with strength s (new; the FreeCT key
 // adaptive_filtration_s is parsed but unused): per view, samples above
 // (1 - s) vmax are blended with their 3 x 3 (row, channel) mean.

@@ -80,50 +80,48 @@ def test_stage_dot_products(ctx):
         assert torch.isfinite(rb).all()
 
 
-def test_whole_chain_dot_products_16_pairs(ctx):
-    # The tiny geometry (96 channels, 8 rows of 5 mm, 12.5 mm voxels) puts
-    # few samples in each voxel and ramp-filters random lines with heavy
-    # cancellation, so FP32 round-off reaches ~1.3e-5 relative here; the 1e-5
-    # bar is held on the adjoint workload (test_adjoint_workload_dot_products).
-    w = _tiny()
+def _chain_dot_test(ctx, w, seed, tag):
+    """<A p, x> against <p, A^T x> over 16 seeded Gaussian pairs, FP64 inner
+    products of the FP32 results.  The pairs are Gaussian with mean 1 and
+    unit variance: with zero-mean pairs <A p, x> is itself a zero-mean sum,
+    the smallest of 16 falls near zero and |a - b| / |a| measures FP32
+    round-off divided by that accident rather than how exactly A^T transposes
+    A.  The zero-mean pairs' error against the Cauchy-Schwarz scale
+    ||A p|| ||x|| is reported beside."""
     p = _plan(ctx, w)
-    gen = torch.Generator(device="cuda").manual_seed(2024)
-    worst = 0.0
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    worst, worst_cs = 0.0, 0.0
+    shp_raw = (w.scan.n_views, w.scan.n_rows, w.scan.n_channels)
+    shp_x = (p.info.n_slices_native, w.recon.ny, w.recon.nx)
     for trial in range(16):
-        raw = torch.randn((w.scan.n_views, w.scan.n_rows, w.scan.n_channels), device="cuda",
-                          generator=gen)
-        x = torch.randn((p.info.n_slices_native, w.recon.ny, w.recon.nx), device="cuda",
-                        generator=gen)
-        ax = p.alloc_native()
-        p.forward_native(raw, ax)
-        atx = p.alloc_raw()
-        p.adjoint(x, atx)
-        torch.cuda.synchronize()
-        a, b = _dot(ax, x), _dot(raw, atx)
-        worst = max(worst, _rel(a, b))
-    say("whole-chain dot test: worst rel %.3g over 16 pairs" % worst)
-    assert worst < 3e-5
+        for mean in (1.0, 0.0):
+            raw = torch.randn(shp_raw, device="cuda", generator=gen) + mean
+            x = torch.randn(shp_x, device="cuda", generator=gen) + mean
+            ax = p.alloc_native()
+            p.forward_native(raw, ax)
+            atx = p.alloc_raw()
+            p.adjoint(x, atx)
+            torch.cuda.synchronize()
+            a, b = _dot(ax, x), _dot(raw, atx)
+            if mean:
+                worst = max(worst, _rel(a, b))
+            else:
+                worst_cs = max(worst_cs, abs(a - b) / (float(ax.double().norm()) *
+                                                       float(x.double().norm())))
+    say("%s dot test: worst rel %.3g over 16 pairs (zero-mean pairs: %.3g of ||Ap|| ||x||)"
+        % (tag, worst, worst_cs))
+    assert worst < 1e-5
+    assert worst_cs < 1e-7
+
+
+def test_whole_chain_dot_products_16_pairs(ctx):
+    _chain_dot_test(ctx, _tiny(), 2024, "whole-chain")
 
 
 def test_adjoint_workload_dot_products(ctx):
-    """888x64 detector, 3 rotations (2952 views), 512^2 x 64 native slices."""
-    w = S.adjoint()
-    p = _plan(ctx, w)
-    gen = torch.Generator(device="cuda").manual_seed(77)
-    worst = 0.0
-    for trial in range(16):
-        raw = torch.randn((w.scan.n_views, w.scan.n_rows, w.scan.n_channels), device="cuda",
-                          generator=gen)
-        x = torch.randn((p.info.n_slices_native, w.recon.ny, w.recon.nx), device="cuda",
-                        generator=gen)
-        ax = p.alloc_native()
-        p.forward_native(raw, ax)
-        atx = p.alloc_raw()
-        p.adjoint(x, atx)
-        torch.cuda.synchronize()
-        worst = max(worst, _rel(_dot(ax, x), _dot(raw, atx)))
-    say("adjoint workload dot test: worst rel %.3g over 16 pairs" % worst)
-    assert worst < 1e-5
+    """The adjoint workload: 736 x 16 detector at 984 views/rot, 3 rotations
+    (2952 views), 512^2 x 64 native slices."""
+    _chain_dot_test(ctx, S.adjoint(), 77, "adjoint workload")
 
 
 def test_adjoint_matches_oracle(ctx, oracle):
@@ -238,3 +236,74 @@ def test_matched_projector_dot_products_and_refinement(ctx):
     x, st = AG.refine(p, raw, x0, steps=4, operator="matched", mask=m)
     say("refine losses", st["loss"])
     assert all(b <= a * (1 + 1e-6) for a, b in zip(st["loss"], st["loss"][1:]))
+
+
+@pytest.mark.parametrize("ffs_mode,rotation", [(0, 1), (0, -1), (3, 1)])
+def test_ray_projector_matches_oracle(ctx, oracle, ffs_mode, rotation):
+    # The ray-driven projector (fct_wfbp_gpu_ray_project) and its transpose
+    # against the FP64 restatement (which traces the physical rays, also the
+    # clockwise and FFS ones, directly), and the GPU dot-product identity.
+    w = S.tiny(nx=40, n_slices=10, n_channels=96, n_rows=8, ppr=96, rotations=1.5,
+               ffs_mode=ffs_mode)
+    sc = w.scan.replace(rotation=rotation, angle0=0.4)
+    rc = w.recon.replace(recon_fov=220.0, y_origin=4.0)
+    from freect_amd import wfbp
+    p = wfbp.Plan(ctx, sc, rc)
+    nz = p.info.n_slices_native
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand((nz, rc.ny, rc.nx), device="cuda", generator=gen)
+    y = torch.rand((sc.n_views, sc.n_rows, sc.n_channels), device="cuda", generator=gen)
+    px, pty = p.alloc_raw(), p.alloc_native()
+    p.ray_project(x, px)
+    p.ray_backproject(y, pty)
+    p.synchronize()
+    ref_px = oracle.ray_project(sc, rc, to_np(x).astype(np.float64))
+    ref_pty = oracle.ray_backproject(sc, rc, to_np(y).astype(np.float64), nz)
+    e1 = np.abs(to_np(px) - ref_px).max() / np.abs(ref_px).max()
+    e2 = np.abs(to_np(pty) - ref_pty).max() / np.abs(ref_pty).max()
+    rel = _rel(_dot(px, y), _dot(x, pty))
+    say("ray projector ffs %d rot %d: P rel %.2e, P^T rel %.2e, dot %.2e" % (ffs_mode, rotation,
+                                                                          e1, e2, rel))
+    assert e1 < 2e-5 and e2 < 2e-5 and rel < 1e-5
+
+
+def test_ray_refinement_lowers_hu_error(ctx):
+    # The data-consistency refinement on a tiny analogue of the adjoint
+    # workload (noiseless, analytic phantom): ten gradient steps on
+    # 0.5 ||m (Pr x - p)||^2 from the WFBP volume, Pr the ray-driven
+    # line-integral projector, step 1 / lambda_max from power iteration.  The
+    # loss falls monotonically and the HU RMSE against the phantom (sampled
+    # at the voxel centres, and averaged over each voxel) falls too.
+    from freect_amd import autograd as AG
+    from freect_amd import wfbp
+    w = S.tiny(n_channels=192, n_rows=16, ppr=192, rotations=3, nx=64, n_slices=16, seed=5)
+    sc, rc = w.scan, w.recon.replace(recon_fov=300.0)
+    p = wfbp.Plan(ctx, sc, rc)
+    raw = ctx.phantom_project(sc, w.phantom, 0.0, w.seed)
+    x0 = AG.wfbp(raw, p).detach()
+    info = p.info
+    nz = info.n_slices_native
+    zs = rc.start_pos + info.collimated_slice_width * np.arange(nz)
+    dx = rc.recon_fov / rc.nx
+    xs = (np.arange(rc.nx) - 0.5 * (rc.nx - 1)) * dx
+    Z, Y, X = np.meshgrid(zs, xs, xs, indexing="ij")
+    pts = w.phantom.mu_at(X, Y, Z)
+    off = (np.arange(3) + 0.5) / 3 - 0.5
+    vox = sum(w.phantom.mu_at(X + a * dx, Y + b * dx, Z + c * info.collimated_slice_width)
+              for a in off for b in off for c in off) / 27.0
+    m3 = ((X * X + Y * Y) <= (0.45 * info.acquisition_fov) ** 2)
+    m3 &= np.isin(np.arange(nz), np.arange(nz // 4, 3 * nz // 4))[:, None, None]
+    HU = S.MU_WATER / 1000.0
+
+    def rmse(x, t):
+        e = (to_np(x).astype(np.float64) - t)[m3] / HU
+        return float(np.sqrt(np.mean(e * e)))
+
+    mask = AG.inside_rays(p)
+    x, st = AG.refine(p, raw, x0, steps=10, operator="ray", mask=mask)
+    r0 = (rmse(x0, pts), rmse(x0, vox))
+    r1 = (rmse(x, pts), rmse(x, vox))
+    say("ray refinement: HU RMSE vs phantom %.2f -> %.2f (voxel-averaged %.2f -> %.2f); "
+        "losses %s" % (r0[0], r1[0], r0[1], r1[1], ["%.4g" % v for v in st["loss"]]))
+    assert all(b <= a * (1 + 1e-6) for a, b in zip(st["loss"], st["loss"][1:]))
+    assert r1[0] < r0[0] and r1[1] < r0[1] - 1.0
