@@ -21,8 +21,8 @@ LIB_PATH = os.path.join(LIBDIR, LIB_NAME)
 ARCH = os.environ.get("FCT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["capi.hip", "rebin_filter.hip", "backproject.hip", "backproject_v1.hip",
-           "aux_kernels.hip", "adjoint.hip", "batch.hip", "adaptive.hip",
-           "rayproj.hip"]
+           "aux_kernels.hip", "adjoint.hip", "batch.hip", "noalloc.hip",
+           "adaptive.hip", "rayproj.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",

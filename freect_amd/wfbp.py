@@ -88,6 +88,7 @@ SIGNATURES = {
     "fct_wfbp_gpu_thicken": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_reconstruct": (C.c_int, [_VP, _VP, _VP, C.c_int32]),
     "fct_wfbp_gpu_reconstruct_stream": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_float, _VP]),
+    "fct_wfbp_gpu_reconstruct_noalloc": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_set_output_units": (C.c_int, [_VP, C.c_int32, C.c_double]),
     "fct_wfbp_gpu_ray_project": (C.c_int, [_VP, _VP, _VP]),
     "fct_wfbp_gpu_ray_backproject": (C.c_int, [_VP, _VP, _VP]),
@@ -397,6 +398,13 @@ class Plan:
         """Units of the thickened volume: "mu" (1/mm, default) or "hu"."""
         self.ctx.check(load_library().fct_wfbp_gpu_set_output_units(
             self.h, {"mu": 0, "hu": 1}[units], float(mu_water)), "set_output_units")
+
+    def reconstruct_noalloc(self, raw, volume):
+        """Whole chain on device buffers without any device allocation (the
+        library's fixed per-GPU scratch; FctError with FCT_ERR_LIMIT when the
+        plan does not fit it)."""
+        self.ctx.check(load_library().fct_wfbp_gpu_reconstruct_noalloc(
+            self.h, self._raw(raw), self._volume(volume)), "reconstruct_noalloc")
 
     def _host_volume(self, out):
         shape = (self.info.n_slices_requested, self.recon.ny, self.recon.nx)

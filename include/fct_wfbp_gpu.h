@@ -224,6 +224,18 @@ FCT_API int fct_wfbp_gpu_set_output_units(fct_wfbp_gpu_plan* plan, int32_t units
 FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* plan, const float* raw, float* volume,
                                      int32_t host_buffers);
 
+/* Whole chain on device buffers (raw [view][row][ch], volume
+ * [slice][y][x]) that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels as fct_wfbp_gpu_reconstruct, with the filtered projections, scale
+ * and native volume in a fixed 256 MiB device array that each GPU holds for
+ * the life of the process.  FCT_ERR_LIMIT when the plan's filtered + native
+ * buffers exceed it (e.g. 888 x 64 x 9840 views); calls that overlap on one
+ * GPU race (they share the array). */
+FCT_API int fct_wfbp_gpu_reconstruct_noalloc(fct_wfbp_gpu_plan* plan, const float* raw,
+                                             float* volume);
+
 /* Source of raw views for the streaming chain: writes views [v0, v0 + n) of
  * the scan, [view][row][ch] float32, to dst (pinned host memory owned by the
  * plan) and returns 0, or nonzero to abort.  Called in view order on the
