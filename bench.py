@@ -692,8 +692,13 @@ def main():
                    "tile_z": chosen.get("tile_z", info.tile_z), "autotune_ms": tuned},
         "s_per_volume": chosen["ms_per_step"] * 1e-3,
         "s_per_volume_e2e": e2e[args.precision],
-        "e2e_note": "scan in pinned host memory, chunked H2D overlapped with rebin+filter and "
-                    "backprojection, volume copied back to pinned host memory",
+        "e2e_note": "scan in pinned host memory; the views some slice reads (%d of %d, views "
+                    "[%d, %d)) uploaded in chunks of 256 on a copy stream, overlapped with "
+                    "rebin+filter and the backprojection of the slab groups whose views have "
+                    "arrived (two streams); slice ranges thickened and copied back to pinned "
+                    "host memory as their groups finish"
+                    % (info.read_views_end - info.read_views_first, w.scan.n_views,
+                       info.read_views_first, info.read_views_end),
         "pcie": pcie,
         "plain_fp32": {"ms_per_step": res["fp32"]["ms_per_step"],
                        "s_per_volume": res["fp32"]["ms_per_step"] * 1e-3,

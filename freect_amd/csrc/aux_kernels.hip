@@ -10,7 +10,7 @@ namespace {
 // sum (FP64 -> FP32).
 __global__ __launch_bounds__(256) void thicken_kernel(ThickenArgs a) {
   const int p = blockIdx.x * 256 + threadIdx.x;
-  const int s = blockIdx.y;
+  const int s = a.s0 + blockIdx.y;
   if (p >= a.plane) return;
   float acc = 0.0f;
   for (int k = a.off[s]; k < a.off[s + 1]; ++k)
@@ -148,6 +148,7 @@ __global__ __launch_bounds__(256) void phantom_kernel(PhantomArgs a) {
 }  // namespace
 
 hipError_t launch_thicken(const ThickenArgs& a, hipStream_t s) {
+  if (a.n_req <= 0) return hipSuccess;
   dim3 grid((a.plane + 255) / 256, a.n_req);
   hipLaunchKernelGGL(thicken_kernel, grid, dim3(256), 0, s, a);
   return hipGetLastError();

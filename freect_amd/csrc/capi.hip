@@ -161,12 +161,18 @@ void free_plan_buffers(fct_wfbp_gpu_plan* p) {
                   p->d_thkT_w, p->d_wrange, p->d_zbounds, p->d_af_raw, p->d_af_vmax};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
-  for (int b = 0; b < 2; ++b) {
+  for (int b = 0; b < 2; ++b)
     if (p->h_stage[b]) (void)hipHostFree(p->h_stage[b]);
-    if (p->ev_stage[b]) (void)hipEventDestroy(p->ev_stage[b]);
-  }
   if (p->ev_compute) (void)hipEventDestroy(p->ev_compute);
   if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
+  for (int b = 0; b < 2; ++b) {
+    if (p->ev_bp[b]) (void)hipEventDestroy(p->ev_bp[b]);
+    if (p->bp_stream[b]) (void)hipStreamDestroy(p->bp_stream[b]);
+  }
+  for (hipEvent_t e : p->ev_chunk) (void)hipEventDestroy(e);
+  if (p->ev_rebin) (void)hipEventDestroy(p->ev_rebin);
+  if (p->ev_ratio) (void)hipEventDestroy(p->ev_ratio);
+  if (p->h_ratio) (void)hipHostFree(p->h_ratio);
 }
 
 // Views a slab of tz slices can select: table z within one feed of the
@@ -297,6 +303,23 @@ fct::RayProjArgs ray_args(const fct_wfbp_gpu_plan* p) {
   return a;
 }
 
+// Thickening of requested slices [s0, s1) from the native slices.
+fct::ThickenArgs thicken_args(const fct_wfbp_gpu_plan* p, const float* native, float* volume,
+                              int s0, int s1) {
+  fct::ThickenArgs a{};
+  a.native = native;
+  a.out = volume;
+  a.off = p->d_thk_off;
+  a.idx = p->d_thk_idx;
+  a.w = p->d_thk_w;
+  a.plane = p->recon.nx * p->recon.ny;
+  a.s0 = s0;
+  a.n_req = s1 - s0;
+  a.hu = p->out_hu;
+  a.hu_scale = p->out_hu ? (float)(1000.0 / p->mu_water) : 1.0f;
+  return a;
+}
+
 // Rebin + filter of every line of the scan (line0 / nlines narrow it).
 fct::RebinFilterArgs rf_args(const fct_wfbp_gpu_plan* p, const float* raw, void* out,
                              float* scale) {
@@ -336,10 +359,18 @@ fct::RebinFilterArgs rf_args(const fct_wfbp_gpu_plan* p, const float* raw, void*
 // and filtered, and groups of slabs whose views are all filtered are
 // backprojected, while later chunks are still being copied.  Each line pair
 // and each tile is computed by the same code as in the one-shot path.
+// Only the views some slab reads travel: the parallel views of the union of
+// the slabs' view ranges and the raw views their rebinning reads (a scan
+// usually extends past the volume by more than the cone reaches).  Slab
+// groups are backprojected on two streams in turn, each group waiting for
+// the rebinning of its last line, so that one group's last waves overlap the
+// next group's first and the rebinning of later chunks.
 // FP16 / BF16 storage needs a bound on max|raw| before the first line is
-// stored: a caller's bound, or the running abs-max of the views so far, in
-// which case the quads already stored are rescaled (by a power of two) when
-// a later chunk raises the storage scale.
+// stored: a caller's bound, or the running abs-max of the views so far.  In
+// the latter case each chunk's scale update is read back (a 4-byte copy the
+// host waits for while the next chunk is in flight) and only when the
+// storage scale grows are the quads already stored rescaled, after the
+// backprojections in flight (which read them) have finished.
 struct ViewSource {
   const float* host;       // caller's scan, [view][row][ch]
   fct_view_reader read;    // or a reader filling pinned staging
@@ -348,6 +379,47 @@ struct ViewSource {
 
 constexpr int kStreamViews = 256;       // default views per chunk
 constexpr int kStreamMinSlabs = 4;      // smallest backprojection group
+constexpr int kStreamMaxSlabs = 10;     // largest backprojection launch
+
+// Parallel views [par.x, par.y) some slab reads, and raw views [raw.x, raw.y)
+// the rebinning of those reads (rebin_sample: projections j + aoff and
+// j + aoff + 1 of the entry's focal state, clamped to [0, mmax]).
+void needed_views(const fct_wfbp_gpu_plan* p, int2* par, int2* raw) {
+  int lo = p->nviews, hi = 0;
+  for (const int2 v : p->h_slab_vrange)
+    if (v.x < v.y) {
+      lo = std::min(lo, v.x);
+      hi = std::max(hi, v.y);
+    }
+  if (lo >= hi) {
+    *par = make_int2(0, 0);
+    *raw = make_int2(0, 0);
+    return;
+  }
+  if (p->nrows & 1) lo &= ~1;   // line pairs start at an even line
+  const int S = p->ffs.S;
+  int rlo = p->raw_scan.n_views, rhi = 0;
+  for (const fct::RebinDev& e : p->h_rebin) {
+    if (e.state == 0) continue;
+    const int rho = e.state - 1;
+    const int ma = std::max(std::min(lo + e.aoff, e.mmax), 0);
+    const int mb = std::max(std::min(hi - 1 + e.aoff + 1, e.mmax), 0);
+    rlo = std::min(rlo, ma * S + rho);
+    rhi = std::max(rhi, mb * S + rho + 1);
+  }
+  if (rlo >= rhi) rlo = rhi = 0;
+  *par = make_int2(lo, hi);
+  *raw = make_int2(rlo, std::min(rhi, p->raw_scan.n_views));
+}
+
+// Page-locked host memory (hipHostMalloc, or registered): copies from and to
+// it are asynchronous DMA; from pageable memory they block the calling thread.
+bool is_pinned(const void* ptr) {
+  hipPointerAttribute_t at{};
+  const hipError_t e = hipPointerGetAttributes(&at, ptr);
+  (void)hipGetLastError();   // clear the error some runtimes report for pageable memory
+  return e == hipSuccess && at.type == hipMemoryTypeHost;
+}
 
 int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
                           float max_abs_raw, float* volume_host) {
@@ -359,6 +431,9 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
   const size_t plane = (size_t)p->recon.ny * p->recon.nx;
   const size_t nat_bytes = (size_t)p->n_native * plane * sizeof(float);
   const size_t vol_bytes = (size_t)p->n_req * plane * sizeof(float);
+  int2 par, rawr;
+  needed_views(p, &par, &rawr);
+  const int nchunks = rawr.y > rawr.x ? (rawr.y - rawr.x + vpc - 1) / vpc : 0;
   if (!p->d_filtered) HIP_TRY(ctx, hipMalloc(&p->d_filtered, p->filtered_bytes()));
   if (p->scale_elems() && !p->d_scale)
     HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&p->d_scale), p->scale_elems() * sizeof(float)));
@@ -368,9 +443,22 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
                            (size_t)p->raw_elems() * sizeof(float)));
   if (!p->d_vol_stage) HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&p->d_vol_stage), vol_bytes));
   if (!p->copy_stream) HIP_TRY(ctx, hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking));
-  for (int b = 0; b < 2; ++b)
-    if (!p->ev_stage[b]) HIP_TRY(ctx, hipEventCreateWithFlags(&p->ev_stage[b], hipEventDisableTiming));
+  for (int b = 0; b < 2; ++b) {
+    if (!p->bp_stream[b])
+      HIP_TRY(ctx, hipStreamCreateWithFlags(&p->bp_stream[b], hipStreamNonBlocking));
+    if (!p->ev_bp[b]) HIP_TRY(ctx, hipEventCreateWithFlags(&p->ev_bp[b], hipEventDisableTiming));
+  }
+  while ((int)p->ev_chunk.size() < nchunks) {
+    hipEvent_t e = nullptr;
+    HIP_TRY(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    p->ev_chunk.push_back(e);
+  }
   if (!p->ev_compute) HIP_TRY(ctx, hipEventCreateWithFlags(&p->ev_compute, hipEventDisableTiming));
+  if (!p->ev_rebin) HIP_TRY(ctx, hipEventCreateWithFlags(&p->ev_rebin, hipEventDisableTiming));
+  if (!p->ev_ratio) HIP_TRY(ctx, hipEventCreateWithFlags(&p->ev_ratio, hipEventDisableTiming));
+  if (!p->h_ratio)
+    HIP_TRY(ctx, hipHostMalloc(reinterpret_cast<void**>(&p->h_ratio), sizeof(float),
+                               hipHostMallocDefault));
   if (src.read && p->stage_views != vpc) {
     for (int b = 0; b < 2; ++b) {
       if (p->h_stage[b]) HIP_TRY(ctx, hipHostFree(p->h_stage[b]));
@@ -382,10 +470,12 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
                                  (size_t)vpc * per_view * sizeof(float), hipHostMallocDefault));
     p->stage_views = vpc;
   }
-  // The copies overwrite the raw buffer only after the plan's earlier work
-  // (which may still read it) has finished.
+  // The copies overwrite the raw buffer, and the backprojections the native
+  // slices, only after the plan's earlier work (which may still read them)
+  // has finished.
   HIP_TRY(ctx, hipEventRecord(p->ev_compute, p->stream));
   HIP_TRY(ctx, hipStreamWaitEvent(p->copy_stream, p->ev_compute, 0));
+  for (int b = 0; b < 2; ++b) HIP_TRY(ctx, hipStreamWaitEvent(p->bp_stream[b], p->ev_compute, 0));
 
   const int prec = p->opts.precision;
   const bool q16 = prec != FCT_PRECISION_FP32;
@@ -395,6 +485,7 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
     if (!grow) {
       HIP_TRY(ctx, fct::launch_scale_set(p->d_scale, max_abs_raw, p->stream));
       HIP_TRY(ctx, fct::launch_scale_update(p->d_scale, (float)p->hsum, p->stream));
+      HIP_TRY(ctx, fct::launch_scale_commit(p->d_scale, p->stream));
     }
   }
   // With adaptive filtration the rebinning reads the filtered copy of each chunk.
@@ -403,26 +494,40 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
   rf.skip_scale = 1;
   fct::BpArgs bp = bp_args(p);
   bp.filt = p->d_filtered;
-  bp.filt_scale = p->d_scale;
+  // scale[3]: scale[0] may change (growth) while a group is in flight.
+  bp.filt_scale = q16 ? p->d_scale + 3 : nullptr;
   bp.native = p->d_native;
-  const int S = p->ffs.S, nrows = p->nrows, npar = p->nviews;
+  const int S = p->ffs.S, nrows = p->nrows;
   const size_t entries_per_view = (size_t)nrows * p->cpitch;
-  int j_done = 0, s_done = 0;
+  int j_done = par.x, s_done = 0, groups = 0, t_done = 0;
+  bool bp_busy[2] = {false, false};
+  // With a pinned source every upload is queued up front, and then volume
+  // ranges go to a pinned destination on the copy stream as they are
+  // thickened; otherwise the whole volume is copied at the end (a
+  // pageable destination synchronously).
+  const bool out_pinned = is_pinned(volume_host);
+  const bool src_pinned = !src.read && is_pinned(src.host);
+  bool out_early = out_pinned && src_pinned;
 
   auto rebin_upto = [&](int j1) -> hipError_t {
     if (j1 <= j_done) return hipSuccess;
-    hipError_t e = hipSuccess;
-    if (grow) {
-      e = fct::launch_scale_update(p->d_scale, (float)p->hsum, p->stream);
-      if (e == hipSuccess)
-        e = fct::launch_quad_rescale(p->d_filtered, (size_t)j_done * entries_per_view, prec,
-                                     p->d_scale, p->ncu, p->stream);
-      if (e != hipSuccess) return e;
-    }
     rf.line0 = j_done * nrows;
     rf.nlines = (j1 - j_done) * nrows;
-    e = fct::launch_rebin_filter(rf, p->stream);
+    hipError_t e = fct::launch_rebin_filter(rf, p->stream);
+    if (e == hipSuccess) e = hipEventRecord(p->ev_rebin, p->stream);
     j_done = j1;
+    return e;
+  };
+  // Requested slices [t_done, t1) whose native slices are all backprojected
+  // are thickened on stream st (and, with out_early, copied to the host
+  // after it on the copy stream, whose uploads are all queued by then).
+  auto thicken_upto = [&](hipStream_t st, int t1) -> hipError_t {
+    if (t1 <= t_done) return hipSuccess;
+    hipError_t e = fct::launch_thicken(thicken_args(p, p->d_native, p->d_vol_stage, t_done, t1), st);
+    if (out_early && e == hipSuccess)
+      e = hipMemcpyAsync(volume_host + (size_t)t_done * plane, p->d_vol_stage + (size_t)t_done * plane,
+                         (size_t)(t1 - t_done) * plane * sizeof(float), hipMemcpyDeviceToHost, st);
+    t_done = t1;
     return e;
   };
   auto bp_ready = [&](bool final) -> hipError_t {
@@ -437,24 +542,72 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
       }
       if (s1 - s_done < kStreamMinSlabs) return hipSuccess;
     }
-    if (s1 <= s_done) return hipSuccess;
-    bp.slab0 = s_done;
-    bp.slab_count = s1 - s_done;
-    s_done = s1;
-    return fct::launch_backproject(bp, p->tile_z, p->variant, prec, fct::kBpAccumulate, p->stream);
+    // The ready slabs in near-equal pieces of at most kStreamMaxSlabs on the
+    // two streams in turn, so that a piece's last waves overlap the next
+    // one's first.
+    const int pieces = (s1 - s_done + kStreamMaxSlabs - 1) / kStreamMaxSlabs;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < pieces && e == hipSuccess; ++i) {
+      const int n = (s1 - s_done + (pieces - i) - 1) / (pieces - i);
+      const int b = groups++ & 1;
+      hipStream_t bs = p->bp_stream[b];
+      e = hipStreamWaitEvent(bs, p->ev_rebin, 0);
+      bp.slab0 = s_done;
+      bp.slab_count = n;
+      s_done += n;
+      if (e == hipSuccess)
+        e = fct::launch_backproject(bp, p->tile_z, p->variant, prec, fct::kBpAccumulate, bs);
+      if (e == hipSuccess) e = hipEventRecord(p->ev_bp[b], bs);
+      bp_busy[b] = true;
+      if (e != hipSuccess || !out_early) continue;
+      // The slices this piece completes, after every piece so far.
+      int t1 = t_done;
+      const int native_done = std::min(p->n_native, s_done * p->tile_z);
+      if (s_done == p->nslabs)
+        t1 = p->n_req;
+      else
+        while (t1 < p->n_req && p->h_thk_need[t1] <= native_done) ++t1;
+      if (t1 > t_done)
+        for (int c = 0; c < 2 && e == hipSuccess; ++c)
+          if (bp_busy[c]) e = hipStreamWaitEvent(p->copy_stream, p->ev_bp[c], 0);
+      if (e == hipSuccess) e = thicken_upto(p->copy_stream, t1);
+    }
+    return e;
+  };
+  // The plan's stream waits for the backprojections issued so far.
+  auto join_bp = [&]() -> hipError_t {
+    for (int b = 0; b < 2; ++b)
+      if (bp_busy[b]) {
+        const hipError_t e = hipStreamWaitEvent(p->stream, p->ev_bp[b], 0);
+        if (e != hipSuccess) return e;
+      }
+    return hipSuccess;
   };
   auto drain = [&]() {
     (void)hipStreamSynchronize(p->copy_stream);
+    for (int b = 0; b < 2; ++b) (void)hipStreamSynchronize(p->bp_stream[b]);
     (void)hipStreamSynchronize(p->stream);
   };
+  auto fail = [&](hipError_t e) {
+    drain();
+    set_err(ctx, "streaming reconstruction: %s", hipGetErrorString(e));
+    return FCT_ERR_DEVICE;
+  };
 
-  for (int v0 = 0, k = 0; v0 < nv; v0 += vpc, ++k) {
-    const int n = std::min(vpc, nv - v0), b = k & 1;
+  // Chunk k covers raw views [rawr.x + k vpc, ...) up to rawr.y.  A caller's
+  // pinned array is copied all at once (the copies queue on the copy
+  // stream); a view reader fills the two staging buffers in turn, and a
+  // pageable array (whose copies block) is copied, one chunk ahead.
+  auto chunk_v0 = [&](int k) { return rawr.x + k * vpc; };
+  auto chunk_n = [&](int k) { return std::min(vpc, rawr.y - chunk_v0(k)); };
+  int issued = 0;
+  auto issue = [&](int k) -> int {
+    const int v0 = chunk_v0(k), n = chunk_n(k), b = k & 1;
     float* dst = p->d_raw_stage + (size_t)v0 * per_view;
     const size_t bytes = (size_t)n * per_view * sizeof(float);
     hipError_t e = hipSuccess;
     if (src.read) {
-      if (k >= 2) e = hipEventSynchronize(p->ev_stage[b]);   // staging b free again
+      if (k >= 2) e = hipEventSynchronize(p->ev_chunk[k - 2]);   // staging b free again
       if (e == hipSuccess && src.read(src.user, v0, n, p->h_stage[b]) != 0) {
         drain();
         set_err(ctx, "the view reader failed for views [%d, %d)", v0, v0 + n);
@@ -466,46 +619,93 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
       e = hipMemcpyAsync(dst, src.host + (size_t)v0 * per_view, bytes, hipMemcpyHostToDevice,
                          p->copy_stream);
     }
-    if (e == hipSuccess) e = hipEventRecord(p->ev_stage[b], p->copy_stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(p->stream, p->ev_stage[b], 0);
-    const int arrived = v0 + n;
-    if (e == hipSuccess && p->af_on)
-      e = fct::launch_adaptive_filter(p->d_raw_stage, p->d_af_raw, p->d_af_vmax, v0, arrived,
-                                      p->raw_scan.n_rows, p->raw_scan.n_channels, p->af_s,
-                                      p->stream);
-    if (e == hipSuccess && grow)
-      e = fct::launch_absmax(rebin_src + (size_t)v0 * per_view, (int64_t)n * (int64_t)per_view,
-                             p->d_scale, p->ncu, p->stream);
+    if (e == hipSuccess) e = hipEventRecord(p->ev_chunk[k], p->copy_stream);
+    if (e != hipSuccess) return fail(e);
+    issued = k + 1;
+    return FCT_OK;
+  };
+
+  // Each step takes every chunk that has landed (at least the next one):
+  // one abs-max / scale update / rebinning launch for all of them, then the
+  // slab groups they make ready.  The scale read-back can wait behind a
+  // running group (whose waves hold the CUs); the chunks that land
+  // meanwhile join the step.
+  bool committed = false;
+  for (int k = 0; k < nchunks;) {
+    const int ahead = src_pinned ? nchunks : std::min(nchunks, k + 2);
+    while (issued < ahead) {
+      const int st = issue(issued);
+      if (st) return st;
+    }
+    hipError_t e = hipEventSynchronize(p->ev_chunk[k]);
+    int k1 = k, arrived = chunk_v0(k);
+    for (;;) {
+      const int ka = k1;
+      k1 = ka + 1;
+      while (e == hipSuccess && k1 < issued && hipEventQuery(p->ev_chunk[k1]) == hipSuccess) ++k1;
+      if (e == hipSuccess) e = hipStreamWaitEvent(p->stream, p->ev_chunk[k1 - 1], 0);
+      const int va = arrived;
+      arrived = chunk_v0(k1 - 1) + chunk_n(k1 - 1);
+      if (e == hipSuccess && p->af_on)
+        e = fct::launch_adaptive_filter(p->d_raw_stage, p->d_af_raw, p->d_af_vmax, va, arrived,
+                                        p->raw_scan.n_rows, p->raw_scan.n_channels, p->af_s,
+                                        p->stream);
+      if (e != hipSuccess || !grow) break;
+      e = fct::launch_absmax(rebin_src + (size_t)va * per_view,
+                             (int64_t)(arrived - va) * (int64_t)per_view, p->d_scale, p->ncu,
+                             p->stream);
+      if (e == hipSuccess) e = fct::launch_scale_update(p->d_scale, (float)p->hsum, p->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(p->h_ratio, p->d_scale + 2, sizeof(float), hipMemcpyDeviceToHost,
+                           p->stream);
+      if (e == hipSuccess) e = hipEventRecord(p->ev_ratio, p->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(p->ev_ratio);
+      const bool grew = *p->h_ratio != 1.0f;
+      if (e == hipSuccess && grew) {
+        // Rescale the stored lines and publish the new scale once no
+        // backprojection reads either.
+        e = join_bp();
+        if (e == hipSuccess && j_done > par.x)
+          e = fct::launch_quad_rescale(
+              static_cast<char*>(p->d_filtered) +
+                  (size_t)par.x * entries_per_view * p->filtered_entry_bytes(),
+              (size_t)(j_done - par.x) * entries_per_view, prec, p->d_scale, p->ncu, p->stream);
+      }
+      if (e == hipSuccess && (grew || !committed)) {
+        e = fct::launch_scale_commit(p->d_scale, p->stream);
+        committed = true;
+      }
+      if (e != hipSuccess || k1 >= issued || hipEventQuery(p->ev_chunk[k1]) != hipSuccess) break;
+    }
     // Parallel views j with j S + reach <= arrived have every raw view they read.
-    int j1 = npar;
-    if (arrived < nv) {
+    int j1 = par.y;
+    if (arrived < rawr.y) {
       const int d = arrived - p->rebin_reach;
-      j1 = d < 0 ? 0 : std::min(npar, d / S + 1);
+      j1 = d < 0 ? 0 : std::min(par.y, d / S + 1);
       if (nrows & 1) j1 &= ~1;   // line pairs must not straddle a chunk end
     }
     if (e == hipSuccess) e = rebin_upto(j1);
     if (e == hipSuccess) e = bp_ready(false);
-    if (e != hipSuccess) {
-      drain();
-      set_err(ctx, "streaming reconstruction: %s", hipGetErrorString(e));
-      return FCT_ERR_DEVICE;
-    }
+    if (e != hipSuccess) return fail(e);
+    k = k1;
   }
-  hipError_t e = rebin_upto(npar);
+  hipError_t e = rebin_upto(par.y);
   if (e == hipSuccess) e = bp_ready(true);
-  if (e != hipSuccess) {
-    drain();
-    set_err(ctx, "streaming reconstruction: %s", hipGetErrorString(e));
-    return FCT_ERR_DEVICE;
+  if (e == hipSuccess) e = join_bp();
+  if (e == hipSuccess && t_done < p->n_req) {
+    // Without out_early (or slices no piece completed): thickened here, and
+    // the whole volume copied below.
+    out_early = false;
+    e = thicken_upto(p->stream, p->n_req);
   }
-  int st = fct_wfbp_gpu_thicken(p, p->d_native, p->d_vol_stage);
-  if (st) {
-    drain();
-    return st;
-  }
-  HIP_TRY(ctx, hipMemcpyAsync(volume_host, p->d_vol_stage, vol_bytes, hipMemcpyDeviceToHost,
-                              p->stream));
+  if (e == hipSuccess && !out_early && out_pinned)
+    e = hipMemcpyAsync(volume_host, p->d_vol_stage, vol_bytes, hipMemcpyDeviceToHost, p->stream);
+  if (e != hipSuccess) return fail(e);
+  HIP_TRY(ctx, hipStreamSynchronize(p->copy_stream));
+  for (int b = 0; b < 2; ++b) HIP_TRY(ctx, hipStreamSynchronize(p->bp_stream[b]));
   HIP_TRY(ctx, hipStreamSynchronize(p->stream));
+  if (!out_pinned)
+    HIP_TRY(ctx, hipMemcpy(volume_host, p->d_vol_stage, vol_bytes, hipMemcpyDeviceToHost));
   return FCT_OK;
 }
 
@@ -745,6 +945,10 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
     }
     toff[so + 1] = (int32_t)tidx.size();
   }
+  p->h_thk_need.assign(p->n_req, 0);
+  for (int so = 0; so < p->n_req; ++so)
+    for (int k = toff[so]; k < toff[so + 1]; ++k)
+      p->h_thk_need[so] = std::max(p->h_thk_need[so], tidx[k] + 1);
   // Transposed table: per native slice, the requested slices and weights.
   std::vector<int32_t> tToff(p->n_native + 1, 0), tTidx;
   std::vector<float> tTw;
@@ -784,6 +988,7 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   }
   PLAN_TRY(upload(&p->d_wrange, wrange, p->stream));
   PLAN_TRY(upload(&p->d_rebin, tab, p->stream));
+  p->h_rebin = tab;
   PLAN_TRY(upload(&p->d_Hn, Hn, p->stream));
   PLAN_TRY(upload(&p->d_tw, tw, p->stream));
   PLAN_TRY(upload(&p->d_thk_off, toff, p->stream));
@@ -835,6 +1040,10 @@ FCT_API int fct_wfbp_gpu_plan_info(const fct_wfbp_gpu_plan* p, fct_plan_info* in
   info->n_filtered_rows = p->nrows;
   info->n_filtered_channels = p->ncols;
   info->ffs_states = p->ffs.S;
+  int2 par, raw;
+  needed_views(p, &par, &raw);
+  info->read_views_first = raw.x;
+  info->read_views_end = raw.y;
   return FCT_OK;
 }
 
@@ -884,17 +1093,7 @@ FCT_API int fct_wfbp_gpu_backproject(fct_wfbp_gpu_plan* p, const void* filtered,
 FCT_API int fct_wfbp_gpu_thicken(fct_wfbp_gpu_plan* p, const float* native, float* volume) {
   if (!p || !native || !volume) return FCT_ERR_INVALID_ARGUMENT;
   HIP_TRY(p->ctx, hipSetDevice(p->device));
-  fct::ThickenArgs a{};
-  a.native = native;
-  a.out = volume;
-  a.off = p->d_thk_off;
-  a.idx = p->d_thk_idx;
-  a.w = p->d_thk_w;
-  a.plane = p->recon.nx * p->recon.ny;
-  a.n_req = p->n_req;
-  a.hu = p->out_hu;
-  a.hu_scale = p->out_hu ? (float)(1000.0 / p->mu_water) : 1.0f;
-  HIP_TRY(p->ctx, fct::launch_thicken(a, p->stream));
+  HIP_TRY(p->ctx, fct::launch_thicken(thicken_args(p, native, volume, 0, p->n_req), p->stream));
   return FCT_OK;
 }
 

@@ -138,6 +138,12 @@ __global__ void scale_update_kernel(float* scale, float hsum) {
   scale[0] = s_new;
 }
 
+// scale[3] = scale[0]: the scale the streaming backprojections read, changed
+// only while none of them runs.
+__global__ void scale_commit_kernel(float* scale) {
+  if (threadIdx.x == 0) scale[3] = scale[0];
+}
+
 // Multiplies n 16-bit values (four per 8-byte entry) by scale[2] when that
 // is not 1: exact for results in the format's normal range.
 template <class T>
@@ -604,6 +610,11 @@ hipError_t launch_absmax(const float* x, int64_t n, float* scale, int ncu, hipSt
 
 hipError_t launch_scale_update(float* scale, float hsum, hipStream_t s) {
   hipLaunchKernelGGL(scale_update_kernel, dim3(1), dim3(64), 0, s, scale, hsum);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_commit(float* scale, hipStream_t s) {
+  hipLaunchKernelGGL(scale_commit_kernel, dim3(1), dim3(64), 0, s, scale);
   return hipGetLastError();
 }
 

@@ -32,7 +32,7 @@ struct RebinDev {
 //    power of two per scan, chosen from a bound on |y| so that |y| / s < 2^15
 //    (no overflow); it is kept in scale[0] of the scan's scale buffer.
 // [0] s, [1] max |raw| bits (rebin+filter scratch), [2] rescale factor of the
-// streaming reconstruction, [3] unused.
+// streaming reconstruction, [3] the s its backprojections read.
 constexpr int kScaleElems = 4;
 
 // Channel weight of the FP16 tap interpolation: the fraction wp rounded to a
@@ -115,7 +115,8 @@ struct ThickenArgs {
   const int32_t* idx;        // native slice indices
   const float* w;            // normalised weights
   int32_t plane;             // nx*ny
-  int32_t n_req;
+  int32_t n_req;             // slices [s0, s0 + n_req) of the output
+  int32_t s0;
   int32_t hu;                // 1: store 1000 (mu - mu_water) / mu_water = mu hu_scale - 1000
   float hu_scale;            // 1000 / mu_water
 };
@@ -177,6 +178,7 @@ hipError_t launch_scale_set(float* scale, float max_abs_raw, hipStream_t s);
 // Streaming FP16 / BF16: scale[0] from the bound so far, scale[2] the factor
 // for the quads already stored; then those n_entries quads rescaled by it.
 hipError_t launch_scale_update(float* scale, float hsum, hipStream_t s);
+hipError_t launch_scale_commit(float* scale, hipStream_t s);
 hipError_t launch_quad_rescale(void* quads, size_t n_entries, int precision, const float* scale,
                                int ncu, hipStream_t s);
 hipError_t launch_backproject_adjoint(const BpAdjArgs& a, int tile_z, hipStream_t s,
@@ -269,11 +271,20 @@ struct fct_wfbp_gpu_plan {
   // Streaming reconstruction (fct_wfbp_gpu_reconstruct_stream and the host
   // path of fct_wfbp_gpu_reconstruct), created on first use.
   hipStream_t copy_stream = nullptr;     // H2D copies (default priority)
-  hipEvent_t ev_stage[2] = {nullptr, nullptr};
   hipEvent_t ev_compute = nullptr;
   float* h_stage[2] = {nullptr, nullptr};   // pinned staging for a view reader
   int stage_views = 0;
+  // Slab groups are backprojected on two streams in turn, so that one
+  // group's last waves overlap the next group's first.
+  hipStream_t bp_stream[2] = {nullptr, nullptr};
+  hipEvent_t ev_bp[2] = {nullptr, nullptr};
+  hipEvent_t ev_rebin = nullptr;         // after the latest rebinning launch
+  hipEvent_t ev_ratio = nullptr;         // after the storage-scale ratio's copy to h_ratio
+  float* h_ratio = nullptr;              // pinned: scale[2] of the latest scale update
   std::vector<int2> h_slab_vrange;      // host copy of d_slab_vrange
+  std::vector<fct::RebinDev> h_rebin;   // host copy of d_rebin
+  std::vector<int32_t> h_thk_need;      // per requested slice: 1 + its last native slice (0: none)
+  std::vector<hipEvent_t> ev_chunk;     // per chunk: its copy has landed
   int rebin_reach = 0;                  // parallel view j reads raw projections < j S + reach
 
   int64_t raw_elems() const {

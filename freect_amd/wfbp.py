@@ -59,7 +59,8 @@ class PlanInfoC(C.Structure):
                 ("collimated_slice_width", C.c_double), ("acquisition_fov", C.c_double),
                 ("z_rot", C.c_double), ("tile_z", C.c_int32), ("bp_variant", C.c_int32),
                 ("n_parallel_views", C.c_int32), ("n_filtered_rows", C.c_int32),
-                ("n_filtered_channels", C.c_int32), ("ffs_states", C.c_int32)]
+                ("n_filtered_channels", C.c_int32), ("ffs_states", C.c_int32),
+                ("read_views_first", C.c_int32), ("read_views_end", C.c_int32)]
 
 
 class BatchInfoC(C.Structure):

@@ -152,6 +152,10 @@ typedef struct {
   int32_t n_filtered_rows;
   int32_t n_filtered_channels;
   int32_t ffs_states;          /* focal states S (1 without FFS) */
+  /* Raw views [first, end) some native slice reads: the views the streaming
+   * chain uploads (and requests from a view reader). */
+  int32_t read_views_first;
+  int32_t read_views_end;
 } fct_plan_info;
 
 typedef struct {
@@ -240,7 +244,9 @@ FCT_API int fct_wfbp_gpu_reconstruct_noalloc(fct_wfbp_gpu_plan* plan, const floa
  * the scan, [view][row][ch] float32, to dst (pinned host memory owned by the
  * plan) and returns 0, or nonzero to abort.  Called in view order on the
  * thread that called fct_wfbp_gpu_reconstruct_stream, while the previous
- * chunk is being copied and processed.  It takes the place of
+ * chunk is being copied and processed, for the views the volume's slices
+ * read only (the leading and trailing views of a scan longer than the
+ * volume needs are not requested).  It takes the place of
  * RawDataSet::copyProjection (RawDataSet.h:136-139) as AllocateKeyArrays
  * uses it (ReconstructionMachine.cu:88-93). */
 typedef int (*fct_view_reader)(void* user, int32_t v0, int32_t n, float* dst);

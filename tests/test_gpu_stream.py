@@ -6,7 +6,9 @@ run while later chunks are copied.
 Pins: the streamed volume is bitwise the device-resident one (every line
 pair and tile is computed by the same code) for FP32, for FP16 with a given
 bound on max|raw|, for a flying-focal-spot scan, an odd row count and a
-table moving backwards; FP16 with the running bound (the stored projections
+table moving backwards and a scan much longer than the volume (whose
+leading and trailing views, read by no slice, are not requested); FP16 with
+the running bound (the stored projections
 rescaled when a later chunk raises the scale) agrees with the one-shot FP16
 volume to the rounding of values below the format's normal range; the
 small workload streamed matches the FP64 restatement (<= 0.5 HU RMSE,
@@ -39,6 +41,13 @@ def _device_volume(ctx, w, raw_t, precision):
     return to_np(vol)
 
 
+def _check_calls(calls, n_views):
+    # One contiguous, increasing run of views inside the scan.
+    assert calls and calls[0][0] >= 0 and calls[-1][0] + calls[-1][1] <= n_views
+    for (a, n), (b, _) in zip(calls, calls[1:]):
+        assert n > 0 and b == a + n
+
+
 def _reader(raw):
     calls = []
 
@@ -63,7 +72,7 @@ def test_small_workload_streamed_matches_device_and_oracle(ctx, oracle, precisio
         read, calls = _reader(raw)
         got = p.reconstruct_stream(read, views_per_chunk=vpc, max_abs_raw=bound)
         n_chunks = len(calls)
-        assert sum(n for _, n in calls) == sc.n_views and calls[0][0] == 0
+        _check_calls(calls, sc.n_views)
         say(f"stream {precision} vpc {vpc}: {n_chunks} chunks, max|diff| "
             f"{np.abs(got - ref).max():.3g}")
         assert np.array_equal(got, ref)
@@ -71,6 +80,11 @@ def test_small_workload_streamed_matches_device_and_oracle(ctx, oracle, precisio
     assert np.array_equal(host, ref)
     pinned = torch.from_numpy(raw).pin_memory()
     host = p.reconstruct_host(pinned)           # pinned: asynchronous DMA
+    assert np.array_equal(host, ref)
+    # Pinned source and destination: every upload queued at once, the volume
+    # copied back in slice ranges as the slab groups finish.
+    out = torch.full(ref.shape, float("nan")).pin_memory()
+    host = p.reconstruct_host(pinned, out=out.numpy())
     assert np.array_equal(host, ref)
     if precision == "fp32":
         d = oracle.geometry(sc, rc)
@@ -104,10 +118,12 @@ def test_running_bound_rescales_stored_projections(ctx, precision):
     assert diff.max() <= 1e-5 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("case", ["ffs", "odd_rows", "backwards"])
+@pytest.mark.parametrize("case", ["ffs", "odd_rows", "backwards", "long_scan"])
 def test_streamed_equals_device_special_scans(ctx, case):
     from freect_amd import wfbp
-    if case == "ffs":
+    if case == "long_scan":   # 6 rotations (+-120 mm) around a 40 mm volume
+        w = S.tiny(nx=40, n_slices=8, n_channels=128, n_rows=8, ppr=128, rotations=6)
+    elif case == "ffs":
         w = S.tiny(nx=48, n_slices=12, n_channels=96, n_rows=8, ppr=192, rotations=3,
                    ffs_mode=3)
     elif case == "odd_rows":
@@ -124,8 +140,26 @@ def test_streamed_equals_device_special_scans(ctx, case):
         bound = float(np.abs(raw).max()) if precision == "fp16" else 0.0
         read, calls = _reader(raw)
         got = p.reconstruct_stream(read, views_per_chunk=37, max_abs_raw=bound)
-        say(f"{case} {precision}: {len(calls)} chunks, max|diff| {np.abs(got - ref).max():.3g}")
+        say(f"{case} {precision}: {len(calls)} chunks, views [{calls[0][0]}, "
+            f"{calls[-1][0] + calls[-1][1]}) of {w.scan.n_views}, "
+            f"max|diff| {np.abs(got - ref).max():.3g}")
+        _check_calls(calls, w.scan.n_views)
+        # Exactly the views some slice reads are requested.
+        assert (calls[0][0], calls[-1][0] + calls[-1][1]) == \
+            (p.info.read_views_first, p.info.read_views_end)
+        if case == "long_scan":   # the views no slice reads are not requested
+            assert calls[0][0] > 0 and calls[-1][0] + calls[-1][1] < w.scan.n_views
         assert np.array_equal(got, ref)
+    if case == "long_scan":   # FP16 with the running bound of the requested views only
+        ref = _device_volume(ctx, w, raw_t, "fp16")
+        p = wfbp.Plan(ctx, w.scan, w.recon, precision="fp16")
+        for pin in (False, True):
+            src = torch.from_numpy(raw).pin_memory() if pin else raw
+            out = torch.empty(ref.shape).pin_memory().numpy() if pin else None
+            got = p.reconstruct_host(src, out=out)
+            diff = np.abs(got.astype(np.float64) - ref)
+            say(f"long_scan fp16 running bound (pinned {pin}): max|diff| {diff.max():.3g}")
+            assert diff.max() <= 1e-5 * np.abs(ref).max()
 
 
 def test_reader_failure_is_an_error_and_plan_stays_usable(ctx):
