@@ -42,31 +42,32 @@ def log(*a):
 
 
 def time_path(plan, raw, steps, warmup, torch):
-    filt, scale = plan.alloc_filtered()
-    native = plan.alloc_native()
+    """One plan's whole chain (fct_wfbp_gpu_reconstruct on device buffers:
+    rebin+filter of the lines some slab reads, backprojection, thickening)
+    per step; then the backprojection alone, timed with HIP events on the
+    plan's stream over the filtered projections of a stage call."""
     vol = plan.alloc_volume()
     stream = torch.cuda.current_stream()
-
-    def step(ev=None):
-        plan.rebin_filter(raw, filt, scale)
-        if ev is not None:
-            ev[0].record(stream)
-        plan.backproject(filt, scale, native)
-        if ev is not None:
-            ev[1].record(stream)
-        plan.thicken(native, vol)
-
     for _ in range(warmup):
-        step()
+        plan.reconstruct(raw, vol)
     torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(steps)]
     t0 = time.perf_counter()
-    for i in range(steps):
-        step(evs[i])
+    for _ in range(steps):
+        plan.reconstruct(raw, vol)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    filt, scale = plan.alloc_filtered()
+    native = plan.alloc_native()
+    plan.rebin_filter(raw, filt, scale)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    for a, b in evs:
+        a.record(stream)
+        plan.backproject(filt, scale, native)
+        b.record(stream)
+    torch.cuda.synchronize()
     bp_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    del filt, scale, native
     return (t1 - t0) * 1e3 / steps, bp_ms, vol
 
 

@@ -135,7 +135,7 @@ int enqueue_backproject(fct_wfbp_gpu_batch* b) {
 int enqueue_rebin_filter(fct_wfbp_gpu_batch* b) {
   for (size_t j = 0; j < b->plans.size(); ++j) {
     fct_wfbp_gpu_plan* p = b->plans[j];
-    const int st = fct_wfbp_gpu_rebin_filter(p, b->raw[j], p->d_filtered, p->d_scale);
+    const int st = fct::chain_rebin_filter(p, b->raw[j], p->d_filtered, p->d_scale);
     if (st != FCT_OK) return st;
   }
   return FCT_OK;

@@ -412,6 +412,43 @@ void needed_views(const fct_wfbp_gpu_plan* p, int2* par, int2* raw) {
   *raw = make_int2(rlo, std::min(rhi, p->raw_scan.n_views));
 }
 
+}  // namespace
+
+namespace fct {
+
+// Rebin + filter of the whole chain: the lines some slab reads only, the
+// FP16 / BF16 bound from the raw views they read (the stage call
+// fct_wfbp_gpu_rebin_filter filters every line of the scan).
+int chain_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, void* filtered, float* scale) {
+  int2 par, rawr;
+  needed_views(p, &par, &rawr);
+  const size_t per_view = (size_t)p->raw_scan.n_rows * p->raw_scan.n_channels;
+  if (p->af_on) {   // adaptive filtration of the views read, into the plan's buffer
+    HIP_TRY(p->ctx, fct::launch_adaptive_filter(raw, p->d_af_raw, p->d_af_vmax, rawr.x, rawr.y,
+                                                p->raw_scan.n_rows, p->raw_scan.n_channels,
+                                                p->af_s, p->stream));
+    raw = p->d_af_raw;
+  }
+  fct::RebinFilterArgs a = rf_args(p, raw, filtered, scale);
+  if (p->opts.precision != FCT_PRECISION_FP32) {
+    HIP_TRY(p->ctx, fct::launch_scale_reset(scale, p->stream));
+    HIP_TRY(p->ctx, fct::launch_absmax(raw + (size_t)rawr.x * per_view,
+                                       (int64_t)(rawr.y - rawr.x) * (int64_t)per_view, scale,
+                                       p->ncu, p->stream));
+    // scale[0] also when no line is left to filter.
+    HIP_TRY(p->ctx, fct::launch_scale_update(scale, (float)p->hsum, p->stream));
+    a.skip_scale = 1;
+  }
+  a.line0 = par.x * p->nrows;
+  a.nlines = (par.y - par.x) * p->nrows;
+  HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
+  return FCT_OK;
+}
+
+}  // namespace fct
+
+namespace {
+
 // Page-locked host memory (hipHostMalloc, or registered): copies from and to
 // it are asynchronous DMA; from pageable memory they block the calling thread.
 bool is_pinned(const void* ptr) {
@@ -1293,7 +1330,7 @@ FCT_API int fct_wfbp_gpu_reconstruct(fct_wfbp_gpu_plan* p, const float* raw, flo
     HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_scale),
                               p->scale_elems() * sizeof(float)));
   if (!p->d_native) HIP_TRY(p->ctx, hipMalloc(reinterpret_cast<void**>(&p->d_native), nat_bytes));
-  int st = fct_wfbp_gpu_rebin_filter(p, raw, p->d_filtered, p->d_scale);
+  int st = fct::chain_rebin_filter(p, raw, p->d_filtered, p->d_scale);
   if (st) return st;
   st = fct_wfbp_gpu_backproject(p, p->d_filtered, p->d_scale, p->d_native);
   if (st) return st;

@@ -86,7 +86,7 @@ FCT_API int fct_wfbp_gpu_reconstruct_noalloc(fct_wfbp_gpu_plan* p, const float* 
   void* filtered = b;
   float* scale = sbytes ? reinterpret_cast<float*>(b + fbytes) : nullptr;
   float* native = reinterpret_cast<float*>(b + fbytes + sbytes);
-  int st = fct_wfbp_gpu_rebin_filter(p, raw, filtered, scale);
+  int st = fct::chain_rebin_filter(p, raw, filtered, scale);
   if (st) return st;
   st = fct_wfbp_gpu_backproject(p, filtered, scale, native);
   if (st) return st;

@@ -211,6 +211,8 @@ namespace fct {
 // Plan helpers shared by capi.hip and batch.hip.
 BpArgs plan_bp_args(const fct_wfbp_gpu_plan* p);   // tables only (no buffers)
 int plan_set_tile_z(fct_wfbp_gpu_plan* p, int tz);  // rebuilds the slab view ranges
+// Rebin + filter of the lines some slab reads (the whole-chain entry points).
+int chain_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, void* filtered, float* scale);
 }  // namespace fct
 
 struct fct_wfbp_gpu_ctx {
