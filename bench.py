@@ -270,7 +270,7 @@ def _cores():
 
 
 def bench_batch(args, torch):
-    """24 uneven jobs on the 888x64 detector: sequential plain FP32 single-plan
+    """24 uneven jobs on the 736x16 detector: sequential plain FP32 single-plan
     reconstructions vs one mixed-precision Batch (graph + persistent BP)."""
     import numpy as np
     from freect_amd import scans as S
@@ -315,16 +315,18 @@ def bench_batch(args, torch):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "storage": args.precision,
         "data": "synthetic (24 seeded jobs, Poisson noise, generated on the GPU)",
-        "config": {"workload": "batch: 24 jobs on 888x64, 2-20 rotations, pitch "
-                               "{0.516,0.984,1.375}, 256^2/512^2, 16-240 slices",
+        "config": {"workload": "batch: 24 jobs on 736x16 (1152 views/rot), 2-20 rotations, "
+                               "pitch {0.516,0.984,1.375}, 256^2/512^2, 16-240 slices cut to "
+                               "the fully covered z-range",
                    "voxel_views": n_vv, "tile_z": b.info()["tile_z"], "autotune_ms": tuned},
         "sequential_plain_fp32": {"ms_per_step": seq_ms,
                                   "voxel_views_per_s": n_vv / (seq_ms * 1e-3)},
         "speedup_vs_sequential_plain_fp32": seq_ms / ms,
-        "roofline": {"bound": "hbm", "kernel": "bp_batch_kernel",
-                     "achieved": n_vv * bpv / (bp_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": n_vv * bpv / (bp_ms * 1e-3) / HBM_PEAK,
-                     "traffic": None, "bp_ms": bp_ms, "bytes_per_voxel_view": bpv},
+        "gather_rate": {"kernel": "bp_batch_kernel", "GBps": n_vv * bpv / (bp_ms * 1e-3) / 1e9,
+                        "bytes_per_voxel_view": bpv, "bp_ms": bp_ms,
+                        "note": "algorithmic tap bytes (served from L1/L2) per backprojection "
+                                "time; not an HBM roofline (the full workload's line carries "
+                                "the counter-measured HBM fraction)"},
         "hu_rmse_batched_vs_sequential_per_job_max": max(hu),
     }
     print(json.dumps(out), flush=True)
@@ -471,15 +473,129 @@ def bench_ffs(args, torch):
         "storage": args.precision,
         "data": "synthetic (seeded ellipsoid phantom through phi+z FFS rays, Poisson I0=2.5e4)",
         "config": {"workload": "ffs: Siemens-like 736x32, 2304 proj/rot phi+z FFS -> 1472x64, "
-                               "pitch 0.8, 8 rotations, 512^2 x 200 slices (1.0 mm at 0.5 mm)",
+                               "pitch 0.8, 8 rotations, 512^2 x 200 slices of 1.0 mm at 1.0 mm "
+                               "over [-100, 100) mm",
                    "voxel_views": n_vv, "n_parallel_views": info.n_parallel_views},
         "plain_fp32": res["fp32"], "speedup_vs_plain_fp32":
             res["fp32"]["ms_per_step"] / chosen["ms_per_step"],
-        "roofline": {"bound": "hbm", "kernel": "bp_v1_kernel",
-                     "achieved": n_vv * bpv / (chosen["bp_ms"] * 1e-3) / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": n_vv * bpv / (chosen["bp_ms"] * 1e-3) / HBM_PEAK, "traffic": None},
+        "gather_rate": {"kernel": "bp_v1_kernel",
+                        "GBps": n_vv * bpv / (chosen["bp_ms"] * 1e-3) / 1e9,
+                        "bytes_per_voxel_view": bpv, "bp_ms": chosen["bp_ms"],
+                        "note": "algorithmic tap bytes (served from L1/L2) per backprojection "
+                                "time; not an HBM roofline"},
         "cpu_baseline": cpu, "hu_error_vs_fp64": hu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_e2e(args, torch):
+    """The full workload end to end from host memory only (the host path of
+    fct_wfbp_gpu_reconstruct: pinned scan in, pinned volume out, streamed),
+    FP32 and the chosen precision, for tracing the streaming schedule."""
+    from freect_amd import scans as S
+    from freect_amd import wfbp
+    w = S.full()
+    ctx = wfbp.Context(0)
+    raw = ctx.phantom_project(w.scan, w.phantom, w.i0, w.seed)
+    torch.cuda.synchronize()
+    raw_host = raw.cpu().pin_memory()
+    del raw
+    res = {}
+    for prec in sorted({"fp32", args.precision}):
+        plan = wfbp.Plan(ctx, w.scan, w.recon, precision=prec)
+        info = plan.info
+        vol = torch.empty((info.n_slices_requested, w.recon.ny, w.recon.nx),
+                          dtype=torch.float32).pin_memory()
+        for _ in range(max(1, args.warmup)):
+            plan.reconstruct_host(raw_host, out=vol.numpy())
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.reconstruct_host(raw_host, out=vol.numpy())
+        res[prec] = (time.perf_counter() - t0) / args.steps
+        log("end to end", prec, res[prec], "s/volume")
+        del plan, vol
+    out = {
+        "metric": "s/volume end to end from pinned host memory", "value": res[args.precision],
+        "unit": "s/volume", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": False, "vs_baseline": None, "storage": args.precision,
+        "data": "synthetic (seeded ellipsoid phantom, Poisson noise I0=5e4, generated on the GPU)",
+        "config": {"workload": "full, streamed: views [%d, %d) of %d uploaded"
+                               % (info.read_views_first, info.read_views_end, w.scan.n_views)},
+        "s_per_volume_e2e": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_toeplitz(args, torch):
+    """The ramp filter of the full workload as one dense Toeplitz product on
+    the matrix cores (hipBLASLt through torch.matmul; contiguous operands) in
+    FP32 / FP16 / BF16, against the register-FFT rebin+filter kernel: time,
+    and the relative error against an FP64 direct convolution of 4096 lines.
+    Lines: the 64 rows of the views the slices read; taps: Ram-Lak at the
+    parallel channel spacing R dgamma."""
+    import numpy as np
+    from freect_amd import scans as S
+    from freect_amd import wfbp
+    w = S.full()
+    sc = w.scan
+    n = sc.n_channels
+    ds = sc.r * sc.dgamma
+    k = np.arange(-(n - 1), n)
+    h = np.zeros(2 * n - 1)
+    h[k == 0] = 1.0 / (4.0 * ds * ds)
+    odd = (k % 2) != 0
+    h[odd] = -1.0 / (np.pi * k[odd] * ds) ** 2
+    T64 = h[np.arange(n)[None, :] - np.arange(n)[:, None] + (n - 1)]   # T[i][c] = h[c - i]
+    ctx = wfbp.Context(0)
+    raw = ctx.phantom_project(sc, w.phantom, w.i0, w.seed)
+    torch.cuda.synchronize()
+    p32 = wfbp.Plan(ctx, sc, w.recon, precision="fp32")
+    lines = (p32.info.read_views_end - p32.info.read_views_first) * sc.n_rows
+    X = raw.reshape(-1, n)[:lines].contiguous()
+    sample = slice(0, 4096)
+    ref = X[sample].double().cpu().numpy() @ T64
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    reps = max(args.steps, 1)
+
+    def timed(fn):
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            out = fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps, out
+
+    gemm = {}
+    for name, dt in (("fp32", torch.float32), ("fp16", torch.float16), ("bf16", torch.bfloat16)):
+        Tt = torch.from_numpy(T64).to("cuda", dt).contiguous()
+        Xt = X.to(dt).contiguous()
+        ms, Y = timed(lambda: torch.matmul(Xt, Tt))
+        got = Y[sample].double().cpu().numpy()
+        gemm[name] = {"ms": ms, "tflops": 2.0 * lines * n * n / (ms * 1e-3) / 1e12,
+                      "rel_rms_err": float(np.sqrt(np.mean((got - ref) ** 2) / np.mean(ref ** 2))),
+                      "rel_max_err": float(np.abs(got - ref).max() / np.abs(ref).max())}
+        log("toeplitz", name, gemm[name])
+        del Tt, Xt, Y
+    fft = {}
+    for prec in ("fp32", "fp16"):
+        p = p32 if prec == "fp32" else wfbp.Plan(ctx, sc, w.recon, precision=prec)
+        filt, scale = p.alloc_filtered()
+        ms, _ = timed(lambda: p.rebin_filter(raw, filt, scale))
+        fft[prec] = {"ms": ms, "lines": sc.n_views * sc.n_rows}
+        log("rebin+filter", prec, fft[prec])
+        del filt, scale
+    out = {
+        "metric": "ramp filter of the full workload: dense Toeplitz GEMM vs register-FFT kernel",
+        "value": gemm["fp32"]["ms"], "unit": "ms (FP32 GEMM)", "n_gpus": 1, "steps": reps,
+        "warmup": args.warmup, "higher_is_better": False, "vs_baseline": None,
+        "data": "synthetic (the full workload's noisy scan rows as filter input)",
+        "config": {"workload": f"toeplitz: {lines} lines x {n} channels, {n} x {n} Toeplitz "
+                               f"Ram-Lak at ds = {ds:.4f} mm"},
+        "gemm": gemm,
+        "rebin_filter_kernel": fft,
     }
     print(json.dumps(out), flush=True)
 
@@ -495,9 +611,12 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--workload", default="full", choices=["full", "batch", "adjoint", "ffs"],
+    ap.add_argument("--workload", default="full",
+                    choices=["full", "batch", "adjoint", "ffs", "toeplitz", "e2e"],
                     help="full: the headline scan (default); batch: 24 uneven jobs; adjoint: "
-                         "the refinement report")
+                         "the refinement report; ffs: the phi+z FFS scan; toeplitz: the ramp "
+                         "filter as a dense GEMM against the FFT kernel; e2e: the full "
+                         "workload from host memory only")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter passes (HBM traffic, VALU activity)")
     ap.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)
@@ -538,6 +657,12 @@ def main():
         return
     if args.workload == "ffs":
         bench_ffs(args, torch)
+        return
+    if args.workload == "toeplitz":
+        bench_toeplitz(args, torch)
+        return
+    if args.workload == "e2e":
+        bench_e2e(args, torch)
         return
     w = S.full()
     ctx = wfbp.Context(0)

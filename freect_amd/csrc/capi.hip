@@ -164,11 +164,8 @@ void free_plan_buffers(fct_wfbp_gpu_plan* p) {
   for (int b = 0; b < 2; ++b)
     if (p->h_stage[b]) (void)hipHostFree(p->h_stage[b]);
   if (p->ev_compute) (void)hipEventDestroy(p->ev_compute);
-  if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
-  for (int b = 0; b < 2; ++b) {
+  for (int b = 0; b < 2; ++b)
     if (p->ev_bp[b]) (void)hipEventDestroy(p->ev_bp[b]);
-    if (p->bp_stream[b]) (void)hipStreamDestroy(p->bp_stream[b]);
-  }
   for (hipEvent_t e : p->ev_chunk) (void)hipEventDestroy(e);
   if (p->ev_rebin) (void)hipEventDestroy(p->ev_rebin);
   if (p->ev_ratio) (void)hipEventDestroy(p->ev_ratio);
@@ -362,9 +359,10 @@ fct::RebinFilterArgs rf_args(const fct_wfbp_gpu_plan* p, const float* raw, void*
 // Only the views some slab reads travel: the parallel views of the union of
 // the slabs' view ranges and the raw views their rebinning reads (a scan
 // usually extends past the volume by more than the cone reaches).  Slab
-// groups are backprojected on two streams in turn, each group waiting for
-// the rebinning of its last line, so that one group's last waves overlap the
-// next group's first and the rebinning of later chunks.
+// groups are backprojected on the context's two backprojection streams in
+// turn, each waiting for the rebinning of its last line, so that they
+// overlap each other's last waves, the rebinning of later chunks and the
+// copies.
 // FP16 / BF16 storage needs a bound on max|raw| before the first line is
 // stored: a caller's bound, or the running abs-max of the views so far.  In
 // the latter case each chunk's scale update is read back (a 4-byte copy the
@@ -479,12 +477,14 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
     HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&p->d_raw_stage),
                            (size_t)p->raw_elems() * sizeof(float)));
   if (!p->d_vol_stage) HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&p->d_vol_stage), vol_bytes));
-  if (!p->copy_stream) HIP_TRY(ctx, hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking));
+  if (!ctx->copy_stream)
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
   for (int b = 0; b < 2; ++b) {
-    if (!p->bp_stream[b])
-      HIP_TRY(ctx, hipStreamCreateWithFlags(&p->bp_stream[b], hipStreamNonBlocking));
+    if (!ctx->bp_stream[b])
+      HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->bp_stream[b], hipStreamNonBlocking));
     if (!p->ev_bp[b]) HIP_TRY(ctx, hipEventCreateWithFlags(&p->ev_bp[b], hipEventDisableTiming));
   }
+  hipStream_t copy_stream = ctx->copy_stream;
   while ((int)p->ev_chunk.size() < nchunks) {
     hipEvent_t e = nullptr;
     HIP_TRY(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -511,8 +511,8 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
   // slices, only after the plan's earlier work (which may still read them)
   // has finished.
   HIP_TRY(ctx, hipEventRecord(p->ev_compute, p->stream));
-  HIP_TRY(ctx, hipStreamWaitEvent(p->copy_stream, p->ev_compute, 0));
-  for (int b = 0; b < 2; ++b) HIP_TRY(ctx, hipStreamWaitEvent(p->bp_stream[b], p->ev_compute, 0));
+  HIP_TRY(ctx, hipStreamWaitEvent(copy_stream, p->ev_compute, 0));
+  for (int b = 0; b < 2; ++b) HIP_TRY(ctx, hipStreamWaitEvent(ctx->bp_stream[b], p->ev_compute, 0));
 
   const int prec = p->opts.precision;
   const bool q16 = prec != FCT_PRECISION_FP32;
@@ -536,7 +536,7 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
   bp.native = p->d_native;
   const int S = p->ffs.S, nrows = p->nrows;
   const size_t entries_per_view = (size_t)nrows * p->cpitch;
-  int j_done = par.x, s_done = 0, groups = 0, t_done = 0;
+  int j_done = par.x, s_done = 0, t_done = 0, pieces_done = 0;
   bool bp_busy[2] = {false, false};
   // With a pinned source every upload is queued up front, and then volume
   // ranges go to a pinned destination on the copy stream as they are
@@ -580,14 +580,15 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
       if (s1 - s_done < kStreamMinSlabs) return hipSuccess;
     }
     // The ready slabs in near-equal pieces of at most kStreamMaxSlabs on the
-    // two streams in turn, so that a piece's last waves overlap the next
-    // one's first.
+    // two backprojection streams in turn, so that a piece's last waves
+    // overlap the next one's first and each piece's slices are thickened and
+    // copied out while the later pieces run.
     const int pieces = (s1 - s_done + kStreamMaxSlabs - 1) / kStreamMaxSlabs;
     hipError_t e = hipSuccess;
     for (int i = 0; i < pieces && e == hipSuccess; ++i) {
       const int n = (s1 - s_done + (pieces - i) - 1) / (pieces - i);
-      const int b = groups++ & 1;
-      hipStream_t bs = p->bp_stream[b];
+      const int b = pieces_done++ & 1;
+      hipStream_t bs = ctx->bp_stream[b];
       e = hipStreamWaitEvent(bs, p->ev_rebin, 0);
       bp.slab0 = s_done;
       bp.slab_count = n;
@@ -606,8 +607,8 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
         while (t1 < p->n_req && p->h_thk_need[t1] <= native_done) ++t1;
       if (t1 > t_done)
         for (int c = 0; c < 2 && e == hipSuccess; ++c)
-          if (bp_busy[c]) e = hipStreamWaitEvent(p->copy_stream, p->ev_bp[c], 0);
-      if (e == hipSuccess) e = thicken_upto(p->copy_stream, t1);
+          if (bp_busy[c]) e = hipStreamWaitEvent(copy_stream, p->ev_bp[c], 0);
+      if (e == hipSuccess) e = thicken_upto(copy_stream, t1);
     }
     return e;
   };
@@ -621,8 +622,8 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
     return hipSuccess;
   };
   auto drain = [&]() {
-    (void)hipStreamSynchronize(p->copy_stream);
-    for (int b = 0; b < 2; ++b) (void)hipStreamSynchronize(p->bp_stream[b]);
+    (void)hipStreamSynchronize(copy_stream);
+    for (int b = 0; b < 2; ++b) (void)hipStreamSynchronize(ctx->bp_stream[b]);
     (void)hipStreamSynchronize(p->stream);
   };
   auto fail = [&](hipError_t e) {
@@ -651,12 +652,12 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
         return FCT_ERR_INVALID_ARGUMENT;
       }
       if (e == hipSuccess)
-        e = hipMemcpyAsync(dst, p->h_stage[b], bytes, hipMemcpyHostToDevice, p->copy_stream);
+        e = hipMemcpyAsync(dst, p->h_stage[b], bytes, hipMemcpyHostToDevice, copy_stream);
     } else {
       e = hipMemcpyAsync(dst, src.host + (size_t)v0 * per_view, bytes, hipMemcpyHostToDevice,
-                         p->copy_stream);
+                         copy_stream);
     }
-    if (e == hipSuccess) e = hipEventRecord(p->ev_chunk[k], p->copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(p->ev_chunk[k], copy_stream);
     if (e != hipSuccess) return fail(e);
     issued = k + 1;
     return FCT_OK;
@@ -738,8 +739,8 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
   if (e == hipSuccess && !out_early && out_pinned)
     e = hipMemcpyAsync(volume_host, p->d_vol_stage, vol_bytes, hipMemcpyDeviceToHost, p->stream);
   if (e != hipSuccess) return fail(e);
-  HIP_TRY(ctx, hipStreamSynchronize(p->copy_stream));
-  for (int b = 0; b < 2; ++b) HIP_TRY(ctx, hipStreamSynchronize(p->bp_stream[b]));
+  HIP_TRY(ctx, hipStreamSynchronize(copy_stream));
+  for (int b = 0; b < 2; ++b) HIP_TRY(ctx, hipStreamSynchronize(ctx->bp_stream[b]));
   HIP_TRY(ctx, hipStreamSynchronize(p->stream));
   if (!out_pinned)
     HIP_TRY(ctx, hipMemcpy(volume_host, p->d_vol_stage, vol_bytes, hipMemcpyDeviceToHost));
@@ -776,6 +777,12 @@ FCT_API int fct_wfbp_gpu_create(int32_t device, fct_wfbp_gpu_ctx** out) {
 }
 
 FCT_API int fct_wfbp_gpu_destroy(fct_wfbp_gpu_ctx* ctx) {
+  if (ctx && (ctx->copy_stream || ctx->bp_stream[0] || ctx->bp_stream[1])) {
+    (void)hipSetDevice(ctx->device);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+    for (int b = 0; b < 2; ++b)
+      if (ctx->bp_stream[b]) (void)hipStreamDestroy(ctx->bp_stream[b]);
+  }
   delete ctx;
   return FCT_OK;
 }

@@ -218,6 +218,13 @@ int chain_rebin_filter(fct_wfbp_gpu_plan* p, const float* raw, void* filtered, f
 struct fct_wfbp_gpu_ctx {
   int device = 0;
   std::string last_error;
+  // The streaming chain's copy stream and two backprojection streams,
+  // shared by the context's plans (created on first use): with a few
+  // hardware queues per process (GPU_MAX_HW_QUEUES), streams of every plan
+  // would share queues with the plans' own streams, and a copy queued
+  // behind a kernel waits for it.
+  hipStream_t copy_stream = nullptr;
+  hipStream_t bp_stream[2] = {nullptr, nullptr};
 };
 
 struct fct_wfbp_gpu_plan {
@@ -272,14 +279,10 @@ struct fct_wfbp_gpu_plan {
 
   // Streaming reconstruction (fct_wfbp_gpu_reconstruct_stream and the host
   // path of fct_wfbp_gpu_reconstruct), created on first use.
-  hipStream_t copy_stream = nullptr;     // H2D copies (default priority)
   hipEvent_t ev_compute = nullptr;
   float* h_stage[2] = {nullptr, nullptr};   // pinned staging for a view reader
   int stage_views = 0;
-  // Slab groups are backprojected on two streams in turn, so that one
-  // group's last waves overlap the next group's first.
-  hipStream_t bp_stream[2] = {nullptr, nullptr};
-  hipEvent_t ev_bp[2] = {nullptr, nullptr};
+  hipEvent_t ev_bp[2] = {nullptr, nullptr};   // after the latest piece on each stream
   hipEvent_t ev_rebin = nullptr;         // after the latest rebinning launch
   hipEvent_t ev_ratio = nullptr;         // after the storage-scale ratio's copy to h_ratio
   float* h_ratio = nullptr;              // pinned: scale[2] of the latest scale update

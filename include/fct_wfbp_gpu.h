@@ -169,7 +169,9 @@ typedef struct fct_wfbp_gpu_ctx fct_wfbp_gpu_ctx;
 typedef struct fct_wfbp_gpu_plan fct_wfbp_gpu_plan;
 
 /* Device selection and validation; replaces validate_selected_device
- * (util.cu:6-35) and main.cpp's -d option (main.cpp:43-44). */
+ * (util.cu:6-35) and main.cpp's -d option (main.cpp:43-44).  The context
+ * owns the streaming chain's copy and backprojection streams, shared by its
+ * plans (streaming calls of several plans of one context queue on them). */
 FCT_API int fct_wfbp_gpu_create(int32_t device, fct_wfbp_gpu_ctx** ctx);
 FCT_API int fct_wfbp_gpu_destroy(fct_wfbp_gpu_ctx* ctx);
 /* Last error message of the context (the reference prints and exits,
