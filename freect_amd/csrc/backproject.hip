@@ -230,6 +230,10 @@ hipError_t launch_tz(const BpArgs& a, int precision, int mode, hipStream_t s) {
 
 hipError_t launch_backproject(const BpArgs& a, int tile_z, int variant, int precision, int mode,
                               hipStream_t s) {
+  // Slabs [slab0, slab0 + slab_count) inside the volume (slab_count 0: every slab).
+  if (a.slab0 < 0 || a.slab_count < 0 || (a.slab_count == 0 && a.slab0 != 0) ||
+      a.slab0 + a.slab_count > a.nslabs)
+    return hipErrorInvalidValue;
   if (variant != FCT_BP_DIRECT && mode != kBpSelect)
     return launch_backproject_v1(a, tile_z, precision, mode, s);
   switch (tile_z) {

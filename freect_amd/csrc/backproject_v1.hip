@@ -752,6 +752,10 @@ hipError_t launch_bp_zbounds(const BpArgs& a, float4* zbounds, hipStream_t s) {
 
 hipError_t launch_backproject_v1(const BpArgs& a, int tile_z, int precision, int mode,
                                  hipStream_t s) {
+  // Slabs [slab0, slab0 + slab_count) inside the volume (slab_count 0: every slab).
+  if (a.slab0 < 0 || a.slab_count < 0 || (a.slab_count == 0 && a.slab0 != 0) ||
+      a.slab0 + a.slab_count > a.nslabs)
+    return hipErrorInvalidValue;
   switch (tile_z) {
     case 4: return launch1<4>(a, precision, mode, s);
     case 8: return launch1<8>(a, precision, mode, s);

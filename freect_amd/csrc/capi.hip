@@ -377,7 +377,6 @@ struct ViewSource {
 
 constexpr int kStreamViews = 256;       // default views per chunk
 constexpr int kStreamMinSlabs = 4;      // smallest backprojection group
-constexpr int kStreamMaxSlabs = 10;     // largest backprojection launch
 
 // Parallel views [par.x, par.y) some slab reads, and raw views [raw.x, raw.y)
 // the rebinning of those reads (rebin_sample: projections j + aoff and
@@ -536,7 +535,7 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
   bp.native = p->d_native;
   const int S = p->ffs.S, nrows = p->nrows;
   const size_t entries_per_view = (size_t)nrows * p->cpitch;
-  int j_done = par.x, s_done = 0, t_done = 0, pieces_done = 0;
+  int j_done = par.x, s_done = 0, t_done = 0, groups = 0;
   bool bp_busy[2] = {false, false};
   // With a pinned source every upload is queued up front, and then volume
   // ranges go to a pinned destination on the copy stream as they are
@@ -579,37 +578,33 @@ int reconstruct_streaming(fct_wfbp_gpu_plan* p, const ViewSource& src, int vpc,
       }
       if (s1 - s_done < kStreamMinSlabs) return hipSuccess;
     }
-    // The ready slabs in near-equal pieces of at most kStreamMaxSlabs on the
-    // two backprojection streams in turn, so that a piece's last waves
-    // overlap the next one's first and each piece's slices are thickened and
-    // copied out while the later pieces run.
-    const int pieces = (s1 - s_done + kStreamMaxSlabs - 1) / kStreamMaxSlabs;
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < pieces && e == hipSuccess; ++i) {
-      const int n = (s1 - s_done + (pieces - i) - 1) / (pieces - i);
-      const int b = pieces_done++ & 1;
-      hipStream_t bs = ctx->bp_stream[b];
-      e = hipStreamWaitEvent(bs, p->ev_rebin, 0);
-      bp.slab0 = s_done;
-      bp.slab_count = n;
-      s_done += n;
-      if (e == hipSuccess)
-        e = fct::launch_backproject(bp, p->tile_z, p->variant, prec, fct::kBpAccumulate, bs);
-      if (e == hipSuccess) e = hipEventRecord(p->ev_bp[b], bs);
-      bp_busy[b] = true;
-      if (e != hipSuccess || !out_early) continue;
-      // The slices this piece completes, after every piece so far.
-      int t1 = t_done;
-      const int native_done = std::min(p->n_native, s_done * p->tile_z);
-      if (s_done == p->nslabs)
-        t1 = p->n_req;
-      else
-        while (t1 < p->n_req && p->h_thk_need[t1] <= native_done) ++t1;
-      if (t1 > t_done)
-        for (int c = 0; c < 2 && e == hipSuccess; ++c)
-          if (bp_busy[c]) e = hipStreamWaitEvent(copy_stream, p->ev_bp[c], 0);
-      if (e == hipSuccess) e = thicken_upto(copy_stream, t1);
-    }
+    if (s1 <= s_done) return hipSuccess;   // every slab already launched
+    // One group of every ready slab per step, on the two backprojection
+    // streams in turn (a group's last waves overlap the next one's first;
+    // splitting the last group so that part of the volume is copied out
+    // earlier made its pieces share the CUs and each other's L2: slower).
+    const int b = groups++ & 1;
+    hipStream_t bs = ctx->bp_stream[b];
+    hipError_t e = hipStreamWaitEvent(bs, p->ev_rebin, 0);
+    bp.slab0 = s_done;
+    bp.slab_count = s1 - s_done;
+    s_done = s1;
+    if (e == hipSuccess)
+      e = fct::launch_backproject(bp, p->tile_z, p->variant, prec, fct::kBpAccumulate, bs);
+    if (e == hipSuccess) e = hipEventRecord(p->ev_bp[b], bs);
+    bp_busy[b] = true;
+    if (e != hipSuccess || !out_early) return e;
+    // The slices this group completes, after every group so far.
+    int t1 = t_done;
+    const int native_done = std::min(p->n_native, s_done * p->tile_z);
+    if (s_done == p->nslabs)
+      t1 = p->n_req;
+    else
+      while (t1 < p->n_req && p->h_thk_need[t1] <= native_done) ++t1;
+    if (t1 > t_done)
+      for (int c = 0; c < 2 && e == hipSuccess; ++c)
+        if (bp_busy[c]) e = hipStreamWaitEvent(copy_stream, p->ev_bp[c], 0);
+    if (e == hipSuccess) e = thicken_upto(copy_stream, t1);
     return e;
   };
   // The plan's stream waits for the backprojections issued so far.
