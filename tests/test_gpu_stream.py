@@ -118,6 +118,39 @@ def test_running_bound_rescales_stored_projections(ctx, precision):
     assert diff.max() <= 1e-5 * np.abs(ref).max()
 
 
+def test_running_bound_growth_with_groups_in_flight(ctx):
+    # 64 slices in slabs of 4 over a 10-rotation helix whose views brighten
+    # 8x along the scan: slab groups start while the scan still streams in,
+    # and the storage scale grows three times after the first group, so each
+    # growth waits for the groups in flight before the stored quads are
+    # rescaled.  Pinned source and destination (every upload queued at once,
+    # slice ranges copied back as groups finish) and a view reader.
+    from freect_amd import wfbp
+    w = S.tiny(nx=32, n_slices=64, n_channels=96, n_rows=8, ppr=96, rotations=10)
+    raw_t = ctx.phantom_project(w.scan, w.phantom, 5e4, w.seed)
+    ramp = torch.logspace(0.0, 3.0, w.scan.n_views, base=2.0, device="cuda")[:, None, None]
+    raw_t = (raw_t * ramp).contiguous()
+    torch.cuda.synchronize()
+    raw = to_np(raw_t)
+    p_dev = wfbp.Plan(ctx, w.scan, w.recon, precision="fp16", tile_z=4)
+    vol = p_dev.alloc_volume()
+    p_dev.reconstruct(raw_t, vol)
+    p_dev.synchronize()
+    ref = to_np(vol)
+    p = wfbp.Plan(ctx, w.scan, w.recon, precision="fp16", tile_z=4)
+    assert p.info.n_slices_native >= 64
+    pinned = torch.from_numpy(raw).pin_memory()
+    out = torch.empty(ref.shape).pin_memory().numpy()
+    got = p.reconstruct_host(pinned, out=out)
+    read, calls = _reader(raw)
+    got2 = p.reconstruct_stream(read, views_per_chunk=32)
+    for name, g in (("pinned", got), ("reader", got2)):
+        diff = np.abs(g.astype(np.float64) - ref)
+        say(f"growth in flight ({name}): max|diff| {diff.max():.3g} of max|vol| "
+            f"{np.abs(ref).max():.3g}")
+        assert diff.max() <= 1e-5 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("case", ["ffs", "odd_rows", "backwards", "long_scan"])
 def test_streamed_equals_device_special_scans(ctx, case):
     from freect_amd import wfbp
