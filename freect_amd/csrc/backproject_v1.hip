@@ -394,6 +394,24 @@ __device__ __forceinline__ void bp_add4(const BpTap<PREC>* s, const ChanInterp<P
   }
 }
 
+// Order of the tiles of a slab: square blocks of kSuper x kSuper tiles
+// (row-major blocks, row-major tiles inside a block; partial blocks at the
+// right and bottom edges), so that the tiles an XCD runs at the same time
+// cover compact patches of the slice rather than a band of whole tile rows:
+// a narrower detector footprint per view in the XCD's L2 (FP16 full
+// workload 52.5 -> 48.6 ms against row-major order; blocks of 8 or 16
+// tiles alike, 4 and 32 slower).  Only the order changes: every tile is
+// computed as before.
+constexpr int kSuper = 16;
+__device__ __forceinline__ int2 tile_of(int l, int tiles_x, int tiles_y) {
+  const int band = kSuper * tiles_x;                 // tiles of a full block row
+  const int sy = l / band, r = l - sy * band;
+  const int h = min(kSuper, tiles_y - sy * kSuper);  // rows of this block row
+  const int sx = r / (kSuper * h), r2 = r - sx * kSuper * h;
+  const int w = min(kSuper, tiles_x - sx * kSuper);  // columns of this block
+  return make_int2(sx * kSuper + r2 % w, sy * kSuper + r2 / w);
+}
+
 // One wave reconstructs tile `lin` (slab-major order) of the job described by a.
 template <int TZ, int PREC, int MODE, bool CENTERED>
 __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
@@ -401,8 +419,9 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   const int lane = threadIdx.x;
   const int tiles = a.tiles_x * a.tiles_y;
   const int slab = lin / tiles;
-  const int tile = lin - slab * tiles;
-  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+  const int2 txy = tile_of(lin - slab * tiles, a.tiles_x, a.tiles_y);
+  const int tx = txy.x, ty = txy.y;
+  const int tile = ty * a.tiles_x + tx;
   const int ix = tx * 8 + (lane & 7), iy = ty * 8 + (lane >> 3);
   const bool inb = ix < a.nx && iy < a.ny;
   const float x = inb ? a.xs[ix] : 0.0f;
