@@ -346,9 +346,22 @@ __device__ __forceinline__ BpTap<PREC> bp_fetch(float zsi, float invLh, float ro
 
 // Weight of a sample (0 when it is not selected): W(q) for the WFBP sum,
 // R / l for the matched projector's plain sum over the views.
-template <int PREC, int MODE>
+//
+// FREE (centred rows, cr == hr, WFBP sum): no selection mask.  The ramp
+// argument t is clamped to [0, 1] by the fma's clamp modifier; the rows a
+// sample may use, qi in [0, nrows - 1), are those with |qi - cr| < hr, and
+// every other qi has |qi - cr| >= hr, where t <= 0 exactly (wk is rounded
+// up), so W = 0 without a compare and a select.  Lanes whose column misses
+// the detector or the FOV enter with row0 = -1e30 and rows per mm 0 (the
+// caller's view setup): t = 0 for them too.  Their tap offsets stay inside
+// the slab (the 24-bit row product of floor(-1e30) is 0) and the taps they
+// read are finite, so num gains 0 * value = 0.
+template <int PREC, int MODE, bool FREE = false>
 __device__ __forceinline__ float bp_weight(const BpTap<PREC>& s, float fR, const FctBpConsts& g) {
   if (MODE == kBpMatched) return s.sel ? fR : 0.0f;
+  if (FREE)
+    return fct_w_of_t(
+        __builtin_amdgcn_fmed3f(fmaf(-fabsf(s.qi - g.cr), g.wk, FCT_WC), 0.0f, 1.0f));
   const float w = fct_w_of_row(s.qi, g);
   return s.sel ? w : 0.0f;
 }
@@ -360,18 +373,18 @@ __device__ __forceinline__ void bp_acc(float W, float val, float& num, float& de
   if (MODE != kBpMatched) den += W;
 }
 
-template <int PREC, int MODE>
+template <int PREC, int MODE, bool FREE = false>
 __device__ __forceinline__ void bp_add(const BpTap<PREC>& s, const ChanInterp<PREC>& ci, float fR,
                                        const FctBpConsts& g, float& num, float& den, int& cnt) {
   if (MODE == kBpCount) {
     cnt += s.sel ? 1 : 0;
     return;
   }
-  bp_acc<MODE>(bp_weight<PREC, MODE>(s, fR, g), ci.value(s.raw, s.fq), num, den);
+  bp_acc<MODE>(bp_weight<PREC, MODE, FREE>(s, fR, g), ci.value(s.raw, s.fq), num, den);
 }
 
 // Four samples of one view (the whole-slab path): FP16 through the dot block.
-template <int PREC, int MODE>
+template <int PREC, int MODE, bool FREE = false>
 __device__ __forceinline__ void bp_add4(const BpTap<PREC>* s, const ChanInterp<PREC>& ci,
                                         float fR, const FctBpConsts& g, float* num, float* den,
                                         int* cnt) {
@@ -383,14 +396,15 @@ __device__ __forceinline__ void bp_add4(const BpTap<PREC>* s, const ChanInterp<P
     for (int i = 0; i < 4; ++i) {
       t[i] = s[i].raw;
       fq[i] = s[i].fq;
-      W[i] = bp_weight<PREC, MODE>(s[i], fR, g);
+      W[i] = bp_weight<PREC, MODE, FREE>(s[i], fR, g);
     }
     ci.values4(t, fq, val);
 #pragma unroll
     for (int i = 0; i < 4; ++i) bp_acc<MODE>(W[i], val[i], num[i], den[i]);
   } else {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) bp_add<PREC, MODE>(s[i], ci, fR, g, num[i], den[i], cnt[i]);
+    for (int i = 0; i < 4; ++i)
+      bp_add<PREC, MODE, FREE>(s[i], ci, fR, g, num[i], den[i], cnt[i]);
   }
 }
 
@@ -453,6 +467,11 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
   // and the slice step have the same sign.  A view whose range lies entirely
   // past the slab in that direction is followed only by such views.
   const bool up = (g.kappa > 0.0f) == (dzs > 0.0f);
+  // Centred WFBP sums: the whole-slab samples take their weight without a
+  // selection mask (bp_weight), the lanes whose column misses the detector
+  // parked at row -1e30 with 0 rows per mm (the per-slice masks of the
+  // partial-slab paths already exclude those lanes).
+  constexpr bool kFree = CENTERED && MODE == kBpAccumulate;
   const bool can_stop = nz > 1;
   const float* ztc = a.zt + (size_t)chunk * a.nviews;
   const int* wrc = reinterpret_cast<const int*>(a.wrange + (size_t)chunk * a.nviews);
@@ -511,10 +530,11 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
       if (k >= kmax) continue;
       const float pidx = par ? fmaf(-p, g.inv_ds, g.cc) : fmaf(p, g.inv_ds, g.cc);
       const float lt = (par ? sq + f : sq - f) * g.tanc;
-      const float invLh = div_rn_normal(g.hr, lt);
       const float kbs = par ? -kb : kb;
       const float pfl = floorf(pidx);
       const bool chan_ok = active && pfl >= 0.0f && pfl <= (float)(g.ncols - 2);
+      // Rows per mm of z; 0 for the lanes parked off the detector (kFree).
+      const float invLh = kFree && !chan_ok ? 0.0f : div_rn_normal(g.hr, lt);
       const uint32_t pf = chan_ok ? (uint32_t)(int)pfl : 0u;
       const float wp = pidx - pfl;
       // Matched mode: R / l, the voxel-driven footprint's depth correction
@@ -545,7 +565,8 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
           if (can_stop && (up ? ilo > nz - 1 : ihi < 0)) break;
           continue;
         }
-        const float row0 = fmaf(kbs - ztv, invLh, g.cr);   // fct_bp_row0
+        const float row0 =                                 // fct_bp_row0
+            kFree && !chan_ok ? -1.0e30f : fmaf(kbs - ztv, invLh, g.cr);
         ChanInterp<PREC> ci;
         ci.init((uint32_t)(v - vr.x) * vstride * kEntry, pf, wp);
         if (ilo == 0 && ihi == nz - 1) {
@@ -553,6 +574,8 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
           // and the taps of four slices are in flight before the first is
           // used (four, not TZ: more would cost waves per SIMD).  With
           // nz < TZ the extra slices repeat the last one and are not stored.
+          // Centred WFBP sums take the weight without a selection mask
+          // (bp_weight; the lanes off the detector are parked at row -1e30).
 #pragma unroll
           for (int h = 0; h < TZ; h += 4) {
             BpTap<PREC> tap[4];
@@ -563,7 +586,7 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
             // FP16: the four loads are issued before the weight arithmetic
             // that waits for them (the scheduler would sink them otherwise).
             if (PREC == FCT_PRECISION_FP16) __builtin_amdgcn_sched_barrier(0);
-            bp_add4<PREC, MODE>(tap, ci, fR, g, num + h, den + h, cnt + h);
+            bp_add4<PREC, MODE, kFree>(tap, ci, fR, g, num + h, den + h, cnt + h);
           }
         } else if constexpr (PREC == FCT_PRECISION_FP16) {
           // Part of the slab, FP16: groups of four slices that meet
@@ -624,16 +647,16 @@ __device__ __forceinline__ void bp_v1_tile(const BpArgs& a, int lin) {
 
 // Occupancy floor for the 16-bit kernels at TZ <= 8: at least seven waves
 // per SIMD (the few spills this costs sit in the per-theta~ setup, outside
-// the sample loop).  At TZ = 16 a floor of six waves spills inside the view
-// loop, so the allocator's own choice (five) stands.
+// the sample loop).  At TZ = 16 five (96 VGPRs; the allocator's own choice
+// is 97, one over); a floor of six spills inside the view loop.
 template <int TZ, int PREC>
 constexpr int bp_min_waves() {
-  return PREC == FCT_PRECISION_FP32 ? 1 : TZ <= 8 ? 7 : 1;
+  return PREC == FCT_PRECISION_FP32 ? 1 : TZ <= 8 ? 7 : TZ == 16 ? 5 : 1;
 }
 
 template <int TZ, int PREC, int MODE, bool CENTERED>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(TZ == 8 ? bp_min_waves<TZ, PREC>() : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(TZ >= 8 ? bp_min_waves<TZ, PREC>() : 1, 8)))
 void bp_v1_kernel(BpArgs a) {
   bp_v1_tile<TZ, PREC, MODE, CENTERED>(
       a, a.slab0 * a.tiles_x * a.tiles_y + xcd_block(blockIdx.x, gridDim.x));

@@ -180,15 +180,19 @@ FCT_HD float fct_w_of_t(float t) {
   return u * (s * s);
 }
 
+// 1 / (1 - Q) as the FP32 expression 1 / (1 - 0.6f) = 2.5000002: t >= 1, so
+// W = 1 exactly, for every FP32 |q| <= 0.6f.
+#define FCT_WC (1.0f / (1.0f - FCT_WQ))
+
 FCT_HD float fct_w_of_q(float q) {
-  return fct_w_of_t(
-      fminf(fmaf(-fabsf(q), 1.0f / (1.0f - FCT_WQ), 1.0f / (1.0f - FCT_WQ)), 1.0f));
+  return fct_w_of_t(fminf(fmaf(-fabsf(q), FCT_WC, FCT_WC), 1.0f));
 }
 
 // The same weight from the row position: |q| = |qi - cr| / hr, so
-// t = min(1/(1-Q) - |qi - cr| * wk, 1) with wk = 1 / (hr (1-Q)).
+// t = min(1/(1-Q) - |qi - cr| * wk, 1) with wk = 1 / (hr (1-Q)), rounded up
+// so that hr * wk >= FCT_WC: t <= 0 wherever |qi - cr| >= hr.
 FCT_HD float fct_w_of_row(float qi, const FctBpConsts& g) {
-  return fct_w_of_t(fminf(fmaf(-fabsf(qi - g.cr), g.wk, 1.0f / (1.0f - FCT_WQ)), 1.0f));
+  return fct_w_of_t(fminf(fmaf(-fabsf(qi - g.cr), g.wk, FCT_WC), 1.0f));
 }
 
 #ifdef __cplusplus
@@ -275,7 +279,11 @@ inline FctBpConsts bp_consts(int ncols, int nrows, double dch, double drow,
   g.cr = (float)cr;
   g.zrot = (float)std::fabs(feed);
   g.half_fov_sq = (float)(0.25 * acq * acq);
+  // Rounded up to the smallest FP32 wk with hr * wk >= FCT_WC (the product
+  // is exact in FP64): the weight ramp then reaches t <= 0 exactly at
+  // |qi - cr| = hr.
   g.wk = (float)(1.0 / (hr * (1.0 - 0.6)));
+  while ((double)g.hr * (double)g.wk < (double)FCT_WC) g.wk = std::nextafter(g.wk, HUGE_VALF);
   g.ncols = ncols;
   g.nrows = nrows;
   return g;

@@ -70,6 +70,7 @@ def lib():
             "or_geometry": (C.c_int, [P(OrScan), P(OrRecon), P(OrDerived)]),
             "or_effective_scan": (None, [P(OrScan), P(OrScan)]),
             "or_weights": (None, [fp, C.c_int, fp, dp]),
+            "or_row_weights": (None, [C.c_int, fp, C.c_int, fp, fp]),
             "or_slice_grid": (C.c_int, [C.c_double, C.c_double, C.c_double, dp, C.c_int]),
             "or_filter_kernel": (None, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, dp]),
             "or_rebin": (None, [P(OrScan), fp, dp, C.c_int, C.c_int]),
@@ -155,6 +156,18 @@ def weights(q):
                      w32.ctypes.data_as(C.POINTER(C.c_float)),
                      w64.ctypes.data_as(C.POINTER(C.c_double)))
     return w32, w64
+
+
+def row_weights(nrows, qi):
+    """The shared FP32 weight from row positions qi for nrows centred rows:
+    (selection-masked fct_w_of_row, unmasked with t clamped to [0, 1])."""
+    qi = np.ascontiguousarray(qi, dtype=np.float32)
+    ws = np.empty(qi.shape, np.float32)
+    wc = np.empty(qi.shape, np.float32)
+    fp = C.POINTER(C.c_float)
+    lib().or_row_weights(int(nrows), qi.ctypes.data_as(fp), int(qi.size), ws.ctypes.data_as(fp),
+                         wc.ctypes.data_as(fp))
+    return ws, wc
 
 
 def effective_scan(scan) -> dict:

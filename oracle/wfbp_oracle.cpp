@@ -269,6 +269,19 @@ void or_weights(const float* q, int n, float* w32, double* w64) {
   }
 }
 
+void or_row_weights(int nrows, const float* qi, int n, float* w_sel, float* w_clamp) {
+  const double cr = 0.5 * ((double)nrows - 1.0);
+  const FctBpConsts g =
+      fct_tables::bp_consts(64, nrows, 1.0, 1.0, 1000.0, 500.0, 31.5, cr, 10.0);
+  for (int i = 0; i < n; ++i) {
+    const float f = std::floor(qi[i]);
+    const bool sel = f >= 0.0f && f <= (float)(nrows - 2);
+    w_sel[i] = sel ? fct_w_of_row(qi[i], g) : 0.0f;
+    const float t = std::fma(-std::fabs(qi[i] - g.cr), g.wk, FCT_WC);
+    w_clamp[i] = fct_w_of_t(std::fmin(std::fmax(t, 0.0f), 1.0f));
+  }
+}
+
 int or_geometry(const or_scan* s, const or_recon* r, or_derived* d) {
   if (!s || !d) return -1;
   const Geo G = make_geo(s);

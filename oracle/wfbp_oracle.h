@@ -81,6 +81,12 @@ typedef struct {
 void or_effective_scan(const or_scan* s, or_scan* out);
 // Shared FP32 helical weight fct_w_of_q and the FP64 definition at q[0..n).
 void or_weights(const float* q, int n, float* w32, double* w64);
+// The shared FP32 weight from the detector row position qi[0..n) for a
+// detector of nrows centred rows (bp_consts): w_sel = fct_w_of_row where the
+// selection rule takes the row pair (floor(qi) in [0, nrows - 2]), else 0;
+// w_clamp = the unmasked form with t clamped to [0, 1] (the backprojection's
+// whole-slab path for centred rows).
+void or_row_weights(int nrows, const float* qi, int n, float* w_sel, float* w_clamp);
 int or_geometry(const or_scan* s, const or_recon* r, or_derived* d);
 int or_slice_grid(double start, double end, double step, double* z, int cap);
 void or_filter_kernel(int n, double ds, int kind, double c, double a, double* h);

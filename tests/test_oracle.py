@@ -136,6 +136,29 @@ def test_fp32_weight_polynomial(oracle):
     assert np.all(np.diff(half) <= 5e-7 * half[1:])
 
 
+def test_row_weight_without_selection_mask(oracle):
+    # The backprojection's whole-slab path for centred rows takes W from the
+    # row position with t clamped to [0, 1] and no selection mask: it relies
+    # on t <= 0 exactly wherever |qi - cr| >= hr (the slope rounded up), so
+    # that every row position the selection rejects weighs exactly 0.  Row
+    # positions on a fine grid, at the two ends of the row range and at their
+    # FP32 neighbours, for even and odd row counts.  Unrounded constants
+    # leave t = +2.4e-7 at the ends (W ~ 1e-13) and fail the exact zero.
+    for nrows in (2, 3, 4, 16, 24, 63, 64, 96, 320):
+        ends = np.array([0.0, nrows - 1.0], np.float32)
+        near = np.concatenate([np.nextafter(ends, np.float32(-np.inf)), ends,
+                               np.nextafter(ends, np.float32(np.inf))])
+        qi = np.concatenate([np.linspace(-3.0, nrows + 2.0, 200001).astype(np.float32),
+                             near, np.float32([-1e30, 1e30])])
+        ws, wc = oracle.row_weights(nrows, qi)
+        off = (qi <= 0.0) | (qi >= nrows - 1.0)
+        assert np.all(wc[off] == 0.0), nrows
+        assert np.all(ws[qi < 0.0] == 0.0) and np.all(ws[qi >= nrows - 1.0] == 0.0)
+        # Where the selection takes the row pair, the two forms agree to the
+        # round-off of t at the ends (W ~ t^2 there).
+        assert np.max(np.abs(wc[~off] - ws[~off])) <= 1e-12, nrows
+
+
 def test_w_special_values_through_backprojection(oracle):
     # W(q) enters only via the weighted mean; a constant filtered sinogram
     # must therefore backproject to scale * (number of covered theta~) exactly.
