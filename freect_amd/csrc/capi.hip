@@ -317,6 +317,14 @@ fct::ThickenArgs thicken_args(const fct_wfbp_gpu_plan* p, const float* native, f
   return a;
 }
 
+// The rebinning gather's raw window (RebinFilterArgs).
+void set_gather_window(const fct_wfbp_gpu_plan* p, fct::RebinFilterArgs* a) {
+  a->group_bytes = (uint32_t)((int64_t)p->ffs.S * p->raw_scan.n_rows * p->raw_scan.n_channels *
+                              (int64_t)sizeof(float));
+  a->aoff_min = p->rebin_aoff_min;
+  a->mmax_min = p->rebin_mmax_min;
+}
+
 // Rebin + filter of every line of the scan (line0 / nlines narrow it).
 fct::RebinFilterArgs rf_args(const fct_wfbp_gpu_plan* p, const float* raw, void* out,
                              float* scale) {
@@ -342,6 +350,7 @@ fct::RebinFilterArgs rf_args(const fct_wfbp_gpu_plan* p, const float* raw, void*
   a.raw_ncols = p->raw_scan.n_channels;
   a.states = p->ffs.S;
   a.zshift = fct_ffs::has_z(p->ffs.mode) ? 1 : 0;
+  set_gather_window(p, &a);
   return a;
 }
 
@@ -933,16 +942,42 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   // with z FFS; without FFS exactly fct_tables::rebin_entry.
   const int zpar = fct_ffs::has_z(p->ffs.mode) ? 2 : 1;
   std::vector<fct::RebinDev> tab((size_t)zpar * p->ncols);
+  const int64_t raw_plane = (int64_t)p->raw_scan.n_rows * p->raw_scan.n_channels;
+  int aoff_max = 0;
+  p->rebin_aoff_min = 0;
+  p->rebin_mmax_min = INT32_MAX;
   for (int zb = 0; zb < zpar; ++zb)
     for (int c = 0; c < p->ncols; ++c) {
       const fct_ffs::Entry e = fct_ffs::rebin_entry(p->ffs, c, zb);
+      // Entries without a fan sample read (state 0, row, channel 0) of a
+      // clamped view and are discarded: their mmax is taken >= 0.
+      const int mmax = std::max(e.mmax, 0);
+      const uint32_t boff =
+          e.valid ? (uint32_t)(((int64_t)e.rho * raw_plane + e.b0) * (int64_t)sizeof(float)) : 0u;
       tab[(size_t)zb * p->ncols + c] =
           fct::RebinDev{e.b0, e.b1, e.aoff, e.valid ? e.rho + 1 : 0, (float)e.fb, (float)e.fa,
-                        e.mmax, 0};
+                        mmax, boff};
+      p->rebin_aoff_min = std::min(p->rebin_aoff_min, e.aoff);
+      aoff_max = std::max(aoff_max, e.aoff);
+      p->rebin_mmax_min = std::min(p->rebin_mmax_min, mmax);
       // Parallel view j reads raw projections up to (j + aoff + 1) S + rho.
       if (e.valid)
         p->rebin_reach = std::max(p->rebin_reach, (e.aoff + 1) * p->ffs.S + e.rho + 1);
     }
+  // The gather addresses a line pair's taps by 32-bit offsets from the first
+  // view group they can reach (rebin_filter.hip): the groups of the table's
+  // aoff span (+ 4: the pair's second view, the interpolation's next view,
+  // the clamp at the last view and the group itself) must fit 4 GiB, and a
+  // view group 16 MiB (24-bit products).
+  {
+    const int64_t group_bytes = (int64_t)p->ffs.S * raw_plane * (int64_t)sizeof(float);
+    if (group_bytes >= (int64_t)1 << 24 ||
+        (int64_t)(aoff_max - p->rebin_aoff_min + 4) * group_bytes >= (int64_t)1 << 32)
+      return fail((set_err(ctx, "a raw view group (%lld bytes) exceeds the rebinning's "
+                                "addressing (16 MiB, 4 GiB per fan)",
+                           (long long)group_bytes),
+                   FCT_ERR_LIMIT));
+  }
 
   // Filter spectrum (generate_filter, ReconstructionMachine.cu:478-544):
   // taps -> fftshift -> DFT in FP64; the spectrum of the symmetric kernel is
@@ -1200,6 +1235,7 @@ FCT_API int fct_wfbp_gpu_rebin(fct_wfbp_gpu_plan* p, const float* raw, float* li
   a.raw_ncols = p->raw_scan.n_channels;
   a.states = p->ffs.S;
   a.zshift = fct_ffs::has_z(p->ffs.mode) ? 1 : 0;
+  set_gather_window(p, &a);
   a.rebin_only = 1;
   HIP_TRY(p->ctx, fct::launch_rebin_filter(a, p->stream));
   return FCT_OK;

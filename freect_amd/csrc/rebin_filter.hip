@@ -63,18 +63,140 @@ __device__ __forceinline__ const RebinDev& rebin_entry_of(const RebinFilterArgs&
   return a.tab[(size_t)(q & a.zshift) * a.ncols + c];
 }
 
-// Rebinned samples of line (parallel view j, row q) at channels t + m T,
-// m = 0..3 (the register kernel's ownership).
-__device__ __forceinline__ void gather_line(const RebinFilterArgs& a, int t, int T, int j, int q,
-                                            float out[4]) {
-  RebinDev e[4];
+// The raw scan as seen by one line pair (views j0 <= j1 <= j0 + 1): a
+// buffer resource based at the first view group any of the pair's taps can
+// reach, m_lo = max(0, min(j0 + aoff_min, mmax_min)) -- every clamped view
+// index max(min(j + aoff, mmax), 0) of the table is >= m_lo -- so that the
+// taps are 32-bit offsets from it for any scan size (the planner checks that
+// the fan's span of view groups fits 4 GiB).
+constexpr int kRawRsrcWord3 = 0x00020000;   // raw buffer, 32-bit data format (gfx9)
+
+struct RawWindow {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int m_lo;
+};
+
+__device__ __forceinline__ RawWindow raw_window(const RebinFilterArgs& a, int j0) {
+  RawWindow w;
+  w.m_lo = max(0, min(j0 + a.aoff_min, a.mmax_min));
+  const size_t base = (size_t)w.m_lo * a.group_bytes;
+  const size_t n = (size_t)a.raw_elems * sizeof(float) - base;
+  w.rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<char*>(reinterpret_cast<const char*>(a.raw) + base), 0,
+      (int)(uint32_t)min(n, (size_t)0xFFFFFFF0u), kRawRsrcWord3);
+  return w;
+}
+
+__device__ __forceinline__ float raw_tap(const RawWindow& w, uint32_t off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(w.rsrc, (int)off, 0, 0));
+}
+
+// rebin_sample through the window: the same clamped views and taps.  jr =
+// j - m_lo; rowB = byte offset of the raw row.  tap[] = (a0, a1, c0, c1):
+// views ma / mb at fan channels b0 / b1.
+struct RebinTaps {
+  float a0, a1, c0, c1;
+};
+
+__device__ __forceinline__ RebinTaps rebin_taps_w(const RawWindow& w, int jr, uint32_t rowB,
+                                                  const RebinDev& e, uint32_t group_bytes) {
+  const int x = jr + e.aoff;
+  const int top = e.mmax - w.m_lo;
+  const uint32_t ma = (uint32_t)max(min(x, top), -w.m_lo);
+  const uint32_t mb = (uint32_t)max(min(x + 1, top), -w.m_lo);
+  const uint32_t o = e.boff + rowB;
+  const uint32_t db = (uint32_t)(e.b1 - e.b0) << 2;
+  const uint32_t oa = __umul24(ma, group_bytes) + o, ob = __umul24(mb, group_bytes) + o;
+  return RebinTaps{raw_tap(w, oa), raw_tap(w, oa + db), raw_tap(w, ob), raw_tap(w, ob + db)};
+}
+
+// The bilinear value of rebin_sample from the taps, as they are (fine when
+// every tap is finite) or with finite_or_zero per tap.
+__device__ __forceinline__ float rebin_combine(const RebinTaps& t, const RebinDev& e) {
+  const float lo = fmaf(e.fa, t.c0 - t.a0, t.a0);
+  const float hi = fmaf(e.fa, t.c1 - t.a1, t.a1);
+  return fmaf(e.fb, hi - lo, lo);
+}
+__device__ __forceinline__ float rebin_combine_finite(const RebinTaps& t, const RebinDev& e) {
+  return rebin_combine(RebinTaps{finite_or_zero(t.a0), finite_or_zero(t.a1),
+                                 finite_or_zero(t.c0), finite_or_zero(t.c1)},
+                       e);
+}
+
+// Plain combinations v[m] of the taps tp[m], recombined with finite_or_zero
+// per tap in lanes where some value is not finite (bitwise the same result:
+// finite_or_zero leaves finite taps alone, and a non-finite tap makes the
+// plain combination non-finite); 0 past ncols and without a fan sample
+// (cuda_kernels.cuh:32-34).
+template <int K>
+__device__ __forceinline__ void rebin_values(const RebinTaps* tp, const RebinDev* const* e,
+                                             const bool* in, float* v) {
+  bool finite = true;
 #pragma unroll
-  for (int m = 0; m < 4; ++m) e[m] = rebin_entry_of(a, q, min(t + m * T, a.ncols - 1));
+  for (int k = 0; k < K; ++k) {
+    v[k] = rebin_combine(tp[k], *e[k]);
+    finite = finite && __builtin_isfinite(v[k]);
+  }
+  if (!finite) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = rebin_combine_finite(tp[k], *e[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = in[k] && e[k]->state ? v[k] : 0.0f;
+}
+
+// Rebinned samples of the pair's lines (j0, q0) and (j1, q1) at channels
+// t + m T, m = 0..3 (the register kernel's ownership); vb = 0 without a
+// second line.  Channels past ncols read the last channel's entry.  Without
+// z FFS (ZFFS false) both lines use the same table entries, read once, and
+// the taps of all eight samples are in flight together; with z FFS the row
+// parities pick different tables and the lines are gathered one after the
+// other.
+template <bool ZFFS>
+__device__ __forceinline__ void gather_pair(const RebinFilterArgs& a, int t, int T, int j0,
+                                            int q0, int j1, int q1, bool has1, float va[4],
+                                            float vb[4]) {
+  const RawWindow w = raw_window(a, j0);
+  const uint32_t rstride = (uint32_t)a.raw_ncols * (uint32_t)sizeof(float);
+  const uint32_t rowB0 = (uint32_t)(q0 >> a.zshift) * rstride;
+  const uint32_t rowB1 = (uint32_t)(q1 >> a.zshift) * rstride;
+  const int jr0 = j0 - w.m_lo, jr1 = (has1 ? j1 : j0) - w.m_lo;
+  RebinDev e[4];
+  bool in[8];
+  const RebinDev* ep[8];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    const float v = rebin_sample(a.raw, j, q, e[m], a);
-    out[m] = t + m * T < a.ncols && e[m].state ? v : 0.0f;
+    e[m] = rebin_entry_of(a, q0, min(t + m * T, a.ncols - 1));
+    in[m] = t + m * T < a.ncols;
+    in[m + 4] = has1 && in[m];
+    ep[m] = ep[m + 4] = &e[m];
   }
+  if (!ZFFS) {
+    RebinTaps tp[8];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      tp[m] = rebin_taps_w(w, jr0, rowB0, e[m], a.group_bytes);
+      tp[m + 4] = rebin_taps_w(w, jr1, rowB1, e[m], a.group_bytes);
+    }
+    float v[8];
+    rebin_values<8>(tp, ep, in, v);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      va[m] = v[m];
+      vb[m] = v[m + 4];
+    }
+    return;
+  }
+  RebinTaps tp[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) tp[m] = rebin_taps_w(w, jr0, rowB0, e[m], a.group_bytes);
+  rebin_values<4>(tp, ep, in, va);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    e[m] = rebin_entry_of(a, q1, min(t + m * T, a.ncols - 1));
+    tp[m] = rebin_taps_w(w, jr1, rowB1, e[m], a.group_bytes);
+  }
+  rebin_values<4>(tp, ep + 4, in + 4, vb);
 }
 
 __device__ FctCplx* fft_forward(FctCplx* a, FctCplx* b, const FctCplx* __restrict__ tw, int N,
@@ -456,17 +578,16 @@ void rebin_filter_reg_kernel(RebinFilterArgs a) {
         v[m + 4] = FctCplx{0.0f, 0.0f};
       }
     } else {
-      // One line at a time, the table entries of the thread's four channels
-      // and then all their taps in flight together (no branch between them;
-      // rebin_sample stays inside the scan for every entry).  Channels past
-      // ncols read the last channel's entry; those and the entries without a
-      // fan sample give 0 (cuda_kernels.cuh:32-34).
+      // The table entries of the thread's four channels and then all their
+      // taps in flight together (no branch between them; every tap offset
+      // lies inside the scan for every entry).  Channels past ncols read the
+      // last channel's entry; those and the entries without a fan sample
+      // give 0 (cuda_kernels.cuh:32-34).
       float va[4], vb[4];
-      gather_line(a, t, T, j0, r0, va);
-      if (has1)
-        gather_line(a, t, T, j1, r1, vb);
+      if (a.zshift)
+        gather_pair<true>(a, t, T, j0, r0, j1, r1, has1, va, vb);
       else
-        vb[0] = vb[1] = vb[2] = vb[3] = 0.0f;
+        gather_pair<false>(a, t, T, j0, r0, j1, r1, has1, va, vb);
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         v[m] = FctCplx{va[m], vb[m]};

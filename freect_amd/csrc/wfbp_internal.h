@@ -20,7 +20,8 @@ namespace fct {
 struct RebinDev {
   int32_t b0, b1, aoff, state;
   float fb, fa;
-  int32_t mmax, pad;
+  int32_t mmax;   // >= 0 (0 for entries without a fan sample)
+  uint32_t boff;  // byte offset of (focal state, raw row 0, fan channel b0) in a view group
 };
 
 // Filtered projections in HBM.
@@ -66,6 +67,10 @@ struct RebinFilterArgs {
   // views as they arrive.
   int32_t line0;
   int32_t skip_scale;        // 1: the FP16 / BF16 bound is already in scale[1]
+  // The gather's raw window (rebinning): bytes per view group (S focal states
+  // x raw rows x raw channels), and the smallest aoff and mmax of the table.
+  uint32_t group_bytes;
+  int32_t aoff_min, mmax_min;
 };
 
 // Everything the backprojection kernel reads.
@@ -291,6 +296,7 @@ struct fct_wfbp_gpu_plan {
   std::vector<int32_t> h_thk_need;      // per requested slice: 1 + its last native slice (0: none)
   std::vector<hipEvent_t> ev_chunk;     // per chunk: its copy has landed
   int rebin_reach = 0;                  // parallel view j reads raw projections < j S + reach
+  int rebin_aoff_min = 0, rebin_mmax_min = 0;   // over the rebinning table (the gather window)
 
   int64_t raw_elems() const {
     return (int64_t)raw_scan.n_views * raw_scan.n_rows * raw_scan.n_channels;
