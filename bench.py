@@ -34,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+XCDS = 8                   # MI355X accelerator complex dies (GRBM_GUI_ACTIVE sums over them)
 BYTES_PER_VV = {"fp32": 16, "fp16": 8, "bf16": 8}   # 4 bilinear taps
 
 
@@ -526,6 +527,27 @@ def bench_e2e(args, torch):
     print(json.dumps(out), flush=True)
 
 
+def bench_valu():
+    """VALU issue rates of this GPU from the probe built beside the library
+    (freect_amd/csrc/probe/valu_probe.hip): independent chains of one wave64
+    instruction, 8 waves per SIMD.  The backprojection's VALU issue
+    (valu_issue_per_simd_cycle in the full workload's roofline) is read
+    against these: a wave64 FP32 instruction holds a 16-lane SIMD for 4
+    cycles (0.25 per SIMD-cycle)."""
+    import subprocess
+    from freect_amd import build as fbuild
+    if not os.path.exists(fbuild.PROBE_PATH):
+        raise SystemExit(f"{fbuild.PROBE_PATH} is missing: run __graft_entry__.build() first")
+    r = subprocess.run([fbuild.PROBE_PATH], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        raise SystemExit(f"valu probe failed ({r.returncode}): {r.stderr.strip()}")
+    probe = json.loads(r.stdout)
+    fma = probe["rates"]["v_fma_f32"]["per_simd_cycle"]
+    print(json.dumps({"metric": "VALU wave64 instructions per SIMD-cycle (v_fma_f32 chains)",
+                      "value": fma, "unit": "instructions/SIMD-cycle", "n_gpus": 1,
+                      "higher_is_better": True, "data": "synthetic", "probe": probe}))
+
+
 def bench_toeplitz(args, torch):
     """The ramp filter of the full workload as one dense Toeplitz product on
     the matrix cores (hipBLASLt through torch.matmul; contiguous operands) in
@@ -612,11 +634,12 @@ def main():
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--workload", default="full",
-                    choices=["full", "batch", "adjoint", "ffs", "toeplitz", "e2e"],
+                    choices=["full", "batch", "adjoint", "ffs", "toeplitz", "e2e", "valu"],
                     help="full: the headline scan (default); batch: 24 uneven jobs; adjoint: "
                          "the refinement report; ffs: the phi+z FFS scan; toeplitz: the ramp "
                          "filter as a dense GEMM against the FFT kernel; e2e: the full "
-                         "workload from host memory only")
+                         "workload from host memory only; valu: the VALU issue rates of this "
+                         "GPU (the backprojection's roofline)")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter passes (HBM traffic, VALU activity)")
     ap.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)
@@ -631,6 +654,10 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
                                    "(multi-GPU z-slab sharding is design-only)"}))
         sys.exit(2)
+
+    if args.workload == "valu":
+        bench_valu()
+        return
 
     import numpy as np
     import torch
@@ -785,11 +812,15 @@ def main():
         roof.update({"achieved": hbm / (chosen["bp_ms"] * 1e-3) / 1e9,
                      "frac": hbm / (chosen["bp_ms"] * 1e-3) / HBM_PEAK, "traffic": hbm,
                      "fetch_bytes": c["fetch_bytes"], "write_bytes": c["write_bytes"],
-                     "valu_busy": c.get("valu_busy"),
+                     "valu_quad_cycle_load": c.get("valu_quad_cycle_load"),
                      "valu_issue_per_simd_cycle": c.get("valu_insts", 0) / (
                          1024 * 2.4e9 * chosen["bp_ms"] * 1e-3),
+                     "valu_issue_peak_fp32_per_simd_cycle": 0.25,
                      "valu_insts_per_voxel_view_wave": c.get("valu_insts", 0) * 64.0 / n_vv,
-                     "limiter": "VALU issue (valu_busy); HBM at frac of peak",
+                     "limiter": "VALU issue (a wave64 FP32 instruction holds a 16-lane SIMD "
+                                "for 4 cycles; the kernel issues above 0.25 per SIMD-cycle "
+                                "only through its faster integer instructions), then the tap "
+                                "gathers' latency; HBM at frac of peak",
                      "counters": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE / SQ_ACTIVE_INST_VALU "
                                  "on a child run of this build (bench.py --probe), calibrated "
                                  "by the abs-max pass (reads the raw scan) and the thickening "
@@ -888,9 +919,14 @@ def measure_counters(args, tile_z, simds=1024):
     FETCH_SIZE, WRITE_SIZE, and SQ_INSTS_VALU + SQ_ACTIVE_INST_VALU +
     GRBM_GUI_ACTIVE.  Calibrated against dispatches of known traffic: the
     abs-max pass reads the raw scan once, the thickening writes the volume
-    once.  valu_busy = 4 SQ_ACTIVE_INST_VALU / (SIMDs x GRBM_GUI_ACTIVE), the
-    fraction of the dispatch's cycles in which the vector ALUs issue.  None if
-    rocprofv3 is unavailable or a pass fails."""
+    once.  valu_quad_cycle_load = 4 SQ_ACTIVE_INST_VALU / (SIMDs x
+    GRBM_GUI_ACTIVE / XCDs): SQ_ACTIVE_INST_VALU counts each VALU instruction
+    at its issue cost in quad-cycles (1 for a wave64 FP32 instruction, 2 for a
+    square root) and GRBM_GUI_ACTIVE reads the sum of the 8 XCDs' cycles, so
+    a dispatch saturated with FP32 instructions reads 1 (0.98 measured on a
+    chain of v_fma_f32); integer additions issue in about 2.5 cycles, so an
+    integer-rich mix can read above 1.  None if rocprofv3 is unavailable or a
+    pass fails."""
     import shutil
     import subprocess
     import tempfile
@@ -940,8 +976,8 @@ def measure_counters(args, tile_z, simds=1024):
         out[name] = {"fetch_bytes": c["FETCH_SIZE"] * fscale,
                      "write_bytes": c["WRITE_SIZE"] * wscale,
                      "valu_insts": c.get("SQ_INSTS_VALU"),
-                     "valu_busy": 4.0 * c["SQ_ACTIVE_INST_VALU"] / (simds * c["GRBM_GUI_ACTIVE"])
-                     if c.get("GRBM_GUI_ACTIVE") else None,
+                     "valu_quad_cycle_load": 4.0 * c["SQ_ACTIVE_INST_VALU"] /
+                     (simds * c["GRBM_GUI_ACTIVE"] / XCDS) if c.get("GRBM_GUI_ACTIVE") else None,
                      "raw": {k: c.get(k) for k in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU",
                                                    "SQ_ACTIVE_INST_VALU", "GRBM_GUI_ACTIVE")}}
     rf = [merged[k] for k in order if "rebin_filter" in k[1]]

@@ -60,6 +60,7 @@ def _compile(src: str) -> str:
 HOST_SOURCES = ["host/fct_host.cpp", "host/dicom.cpp"]
 CLI_SOURCE = "host/main.cpp"
 CLI_PATH = os.path.join(LIBDIR, "fct_wfbp")
+PROBE_PATH = os.path.join(LIBDIR, "fct_valu_probe")
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-ffp-contract=off",
              "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
@@ -94,6 +95,13 @@ def build_library(jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"CLI link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(CLI_PATH + ".tmp", CLI_PATH)
+    # The VALU issue-rate probe (bench.py --workload valu).
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", PROBE_PATH + ".tmp",
+           os.path.join(CSRC, "probe", "valu_probe.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"probe build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(PROBE_PATH + ".tmp", PROBE_PATH)
     return LIB_PATH
 
 
