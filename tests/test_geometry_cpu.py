@@ -3,7 +3,8 @@
 * every address the rebinning gather forms (rebin_filter.hip: rebin_sample,
   with the table as the plan uploads it) lies inside the raw scan, for every
   flying-focal-spot mode and view counts that are not multiples of the focal
-  states;
+  states, and the register kernel's windowed 32-bit offsets address the same
+  taps;
 * a clockwise scan: the FP64 projector against chords computed here from the
   definition (source at angle0 - v dtheta, fan angle measured with the
   rotation), and the host's per-view law check on clockwise and jittered
@@ -62,6 +63,35 @@ def test_rebin_gather_addresses_inside_scan(oracle, ffs_mode, extra_views):
     # Selected entries read the focal state they name.
     sel = valid != 0
     assert np.all(rho_r[sel] == rho[sel]) and np.all(rho[sel] < Sn)
+
+    # The register kernel's windowed form of the same taps (rebin_filter.hip:
+    # raw_window, rebin_taps_w): byte offsets from the first raw view group a
+    # line pair (views j0 <= j1 <= j0 + 1) can reach, m_lo = max(0, min(j0 +
+    # aoff_min, mmax_min)), with the view clamp taken relative to m_lo.  Every
+    # relative view index is >= 0 (24-bit products), every offset fits the
+    # 32-bit window and addresses exactly the tap of the plain formula.
+    group_bytes = Sn * sc.n_rows * sc.n_channels * 4
+    assert group_bytes < 1 << 24
+    boff = np.where(valid != 0, (rho * sc.n_rows * sc.n_channels + b0) * 4, 0)
+    aoff_min, mmax_min = min(int(aoff.min()), 0), int(mmax.min())
+    j0 = np.arange(e.n_views, dtype=np.int64)[:, None]
+    m_lo = np.maximum(0, np.minimum(j0 + aoff_min, mmax_min))
+    for dj in (0, 1):
+        j1 = np.minimum(j0 + dj, e.n_views - 1)
+        for off in (0, 1):
+            m_abs = np.maximum(np.minimum(j1 + aoff[None, :] + off, mmax[None, :]), 0)
+            rel = np.maximum(np.minimum(j1 - m_lo + aoff[None, :] + off, mmax[None, :] - m_lo),
+                             -m_lo)
+            assert np.array_equal(rel, m_abs - m_lo)
+            assert rel.min() >= 0 and rel.max() < 1 << 24
+            for q in range(e.n_rows):
+                row = q >> (1 if zpar == 2 else 0)
+                o0 = rel * group_bytes + boff[None, :] + row * sc.n_channels * 4
+                o1 = o0 + (b1 - b0)[None, :] * 4
+                assert o0.min() >= 0 and max(o0.max(), o1.max()) + 4 <= 0xFFFFFFF0
+                for o, b in ((o0, b0), (o1, b1)):
+                    plain = ((m_abs * Sn + rho_r[None, :]) * sc.n_rows + row) * sc.n_channels
+                    assert np.array_equal(m_lo * group_bytes + o, (plain + b[None, :]) * 4)
 
 
 def _sphere_chords_cw(sc, centre, radius, dmu):
