@@ -71,6 +71,8 @@ __device__ __forceinline__ const RebinDev& rebin_entry_of(const RebinFilterArgs&
 // the fan's span of view groups fits 4 GiB).
 constexpr int kRawRsrcWord3 = 0x00020000;   // raw buffer, 32-bit data format (gfx9)
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 struct RawWindow {
   __amdgpu_buffer_rsrc_t rsrc;
   int m_lo;
@@ -98,16 +100,23 @@ struct RebinTaps {
   float a0, a1, c0, c1;
 };
 
+// The two fan channels of a tap row are adjacent (b1 = b0 + 1) except at the
+// last fan channel (b1 = b0): one 8-byte load per view reads both, starting
+// at b0, or at b0 - 1 for the last channel (inside the row), whose second
+// element is then the tap for both -- the same values as four single loads.
 __device__ __forceinline__ RebinTaps rebin_taps_w(const RawWindow& w, int jr, uint32_t rowB,
                                                   const RebinDev& e, uint32_t group_bytes) {
   const int x = jr + e.aoff;
   const int top = e.mmax - w.m_lo;
   const uint32_t ma = (uint32_t)max(min(x, top), -w.m_lo);
   const uint32_t mb = (uint32_t)max(min(x + 1, top), -w.m_lo);
-  const uint32_t o = e.boff + rowB;
-  const uint32_t db = (uint32_t)(e.b1 - e.b0) << 2;
+  const bool last = e.b1 == e.b0;
+  const uint32_t o = e.boff + rowB - (last ? 4u : 0u);
   const uint32_t oa = __umul24(ma, group_bytes) + o, ob = __umul24(mb, group_bytes) + o;
-  return RebinTaps{raw_tap(w, oa), raw_tap(w, oa + db), raw_tap(w, ob), raw_tap(w, ob + db)};
+  const u32x2 ta = __builtin_amdgcn_raw_buffer_load_b64(w.rsrc, (int)oa, 0, 0);
+  const u32x2 tb = __builtin_amdgcn_raw_buffer_load_b64(w.rsrc, (int)ob, 0, 0);
+  return RebinTaps{__uint_as_float(last ? ta.y : ta.x), __uint_as_float(ta.y),
+                   __uint_as_float(last ? tb.y : tb.x), __uint_as_float(tb.y)};
 }
 
 // The bilinear value of rebin_sample from the taps, as they are (fine when
@@ -326,6 +335,20 @@ __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
          ((uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
 }
 
+// The filtered output is stored non-temporally (streaming, not kept in L2):
+// it is read back only by the backprojection, from HBM, while the raw fan rows
+// the gather reads are each used by the ~150 parallel views that follow, and
+// output lines kept in L2 would evict them first.
+typedef uint32_t nt_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(uint32_t* p, uint32_t v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(uint32_t* p, uint32_t lo, uint32_t hi) {
+  nt_u32x2 v;
+  v.x = lo;
+  v.y = hi;
+  __builtin_nontemporal_store(v, reinterpret_cast<nt_u32x2*>(p));
+}
+
 // Rows r (even) and r + 1 of one view at channel c, entry index e of (r, c),
 // with the values at c and c + 1 of both rows: the whole quad (r, c), the
 // first half of (r + 1, c) (with a zero second half on the last row) and the
@@ -336,13 +359,13 @@ __device__ __forceinline__ void quad_store_pair(const RebinFilterArgs& a, size_t
                                                 float y00, float y01, float y10, float y11) {
   uint32_t* q = static_cast<uint32_t*>(a.out);   // two 32-bit halves per entry
   const uint32_t lo = pack2<T>(y00, y01), hi = pack2<T>(y10, y11);
-  *reinterpret_cast<uint2*>(q + e * 2) = make_uint2(lo, hi);
+  st_nt(q + e * 2, lo, hi);
   const size_t e1 = e + (size_t)a.cpitch;
   if (r + 1 == a.nrows - 1)
-    *reinterpret_cast<uint2*>(q + e1 * 2) = make_uint2(hi, 0u);
+    st_nt(q + e1 * 2, hi, 0u);
   else
-    q[e1 * 2] = hi;
-  if (r > 0) q[(e - (size_t)a.cpitch) * 2 + 1] = lo;
+    st_nt(q + e1 * 2, hi);
+  if (r > 0) st_nt(q + (e - (size_t)a.cpitch) * 2 + 1, lo);
 }
 
 // max|x| over the raw scan as FP32 bits in scale[1] (non-negative floats
@@ -512,7 +535,7 @@ __device__ __forceinline__ void fft_forward_regs(FctCplx* v, FctCplx* lds, int t
 }
 
 template <int N>
-__global__ __launch_bounds__(N / 8)
+__global__ __launch_bounds__(N / 8) __attribute__((amdgpu_waves_per_eu(3, 8)))
 void rebin_filter_reg_kernel(RebinFilterArgs a) {
   using S = FctFftShape<N>;
   constexpr int T = S::T;
@@ -601,8 +624,8 @@ void rebin_filter_reg_kernel(RebinFilterArgs a) {
       for (int m = 0; m < 4; ++m) {
         const int c = t + m * T;
         if (c < a.cpitch) {
-          out[(size_t)l0 * a.cpitch + c] = v[m].x;
-          if (has1) out[(size_t)l1 * a.cpitch + c] = v[m].y;
+          st_nt(out + (size_t)l0 * a.cpitch + c, v[m].x);
+          if (has1) st_nt(out + (size_t)l1 * a.cpitch + c, v[m].y);
         }
       }
       continue;
@@ -625,8 +648,8 @@ void rebin_filter_reg_kernel(RebinFilterArgs a) {
         if (c < a.cpitch) {
           const FctCplx y = w[fct_fft_final_reg<N>(m)];
           const bool in = c < a.ncols;
-          out[o0 + c] = in ? y.x : 0.0f;
-          if (has1) out[o1 + c] = in ? -y.y : 0.0f;
+          st_nt(out + o0 + c, in ? y.x : 0.0f);
+          if (has1) st_nt(out + o1 + c, in ? -y.y : 0.0f);
         }
       }
       continue;
