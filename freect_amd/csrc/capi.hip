@@ -943,7 +943,7 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
   const int zpar = fct_ffs::has_z(p->ffs.mode) ? 2 : 1;
   std::vector<fct::RebinDev> tab((size_t)zpar * p->ncols);
   const int64_t raw_plane = (int64_t)p->raw_scan.n_rows * p->raw_scan.n_channels;
-  int aoff_max = 0;
+  int aoff_max = 0, mmax_max = 0;
   p->rebin_aoff_min = 0;
   p->rebin_mmax_min = INT32_MAX;
   for (int zb = 0; zb < zpar; ++zb)
@@ -960,6 +960,7 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
       p->rebin_aoff_min = std::min(p->rebin_aoff_min, e.aoff);
       aoff_max = std::max(aoff_max, e.aoff);
       p->rebin_mmax_min = std::min(p->rebin_mmax_min, mmax);
+      mmax_max = std::max(mmax_max, mmax);
       // Parallel view j reads raw projections up to (j + aoff + 1) S + rho.
       if (e.valid)
         p->rebin_reach = std::max(p->rebin_reach, (e.aoff + 1) * p->ffs.S + e.rho + 1);
@@ -976,6 +977,13 @@ FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geome
       return fail((set_err(ctx, "a raw view group (%lld bytes) exceeds the rebinning's "
                                 "addressing (16 MiB, 4 GiB per fan)",
                            (long long)group_bytes),
+                   FCT_ERR_LIMIT));
+    // The packed 16-byte entries of the gather (aoff in 12 bits, mmax in 20).
+    if (p->rebin_aoff_min < -2048 || aoff_max > 2047 || mmax_max >= (1 << 20))
+      return fail((set_err(ctx, "the rebinning table exceeds its packed form (fan view "
+                                "offsets %d..%d, last projection %d; limits -2048..2047, "
+                                "2^20 - 1)",
+                           p->rebin_aoff_min, aoff_max, mmax_max),
                    FCT_ERR_LIMIT));
   }
 

@@ -100,17 +100,44 @@ struct RebinTaps {
   float a0, a1, c0, c1;
 };
 
+// What the gather needs of a table entry.  Without z FFS a thread's four
+// entries are the same for every line pair: they are packed once into 16
+// bytes each in LDS (boff | last | valid << 1, aoff in 12 bits | mmax << 12,
+// fa, fb; the planner checks |aoff| < 2048 and mmax < 2^20) and read back
+// with one LDS load per pair instead of two global loads.
+struct GatherEntry {
+  uint32_t boff;
+  int aoff, mmax;
+  float fa, fb;
+  bool last, valid;   // b1 == b0 (last fan channel); a fan sample exists
+};
+
+__device__ __forceinline__ GatherEntry gather_entry(const RebinDev& e) {
+  return GatherEntry{e.boff, e.aoff, e.mmax, e.fa, e.fb, e.b1 == e.b0, e.state != 0};
+}
+
+__device__ __forceinline__ uint4 pack_entry(const RebinDev& e) {
+  return make_uint4(e.boff | (e.b1 == e.b0 ? 1u : 0u) | (e.state ? 2u : 0u),
+                    ((uint32_t)e.aoff & 0xfffu) | ((uint32_t)e.mmax << 12), __float_as_uint(e.fa),
+                    __float_as_uint(e.fb));
+}
+
+__device__ __forceinline__ GatherEntry unpack_entry(uint4 p) {
+  return GatherEntry{p.x & ~3u, (int)(p.y << 20) >> 20, (int)(p.y >> 12), __uint_as_float(p.z),
+                     __uint_as_float(p.w), (p.x & 1u) != 0, (p.x & 2u) != 0};
+}
+
 // The two fan channels of a tap row are adjacent (b1 = b0 + 1) except at the
 // last fan channel (b1 = b0): one 8-byte load per view reads both, starting
 // at b0, or at b0 - 1 for the last channel (inside the row), whose second
 // element is then the tap for both -- the same values as four single loads.
 __device__ __forceinline__ RebinTaps rebin_taps_w(const RawWindow& w, int jr, uint32_t rowB,
-                                                  const RebinDev& e, uint32_t group_bytes) {
+                                                  const GatherEntry& e, uint32_t group_bytes) {
   const int x = jr + e.aoff;
   const int top = e.mmax - w.m_lo;
   const uint32_t ma = (uint32_t)max(min(x, top), -w.m_lo);
   const uint32_t mb = (uint32_t)max(min(x + 1, top), -w.m_lo);
-  const bool last = e.b1 == e.b0;
+  const bool last = e.last;
   const uint32_t o = e.boff + rowB - (last ? 4u : 0u);
   const uint32_t oa = __umul24(ma, group_bytes) + o, ob = __umul24(mb, group_bytes) + o;
   const u32x2 ta = __builtin_amdgcn_raw_buffer_load_b64(w.rsrc, (int)oa, 0, 0);
@@ -121,12 +148,12 @@ __device__ __forceinline__ RebinTaps rebin_taps_w(const RawWindow& w, int jr, ui
 
 // The bilinear value of rebin_sample from the taps, as they are (fine when
 // every tap is finite) or with finite_or_zero per tap.
-__device__ __forceinline__ float rebin_combine(const RebinTaps& t, const RebinDev& e) {
+__device__ __forceinline__ float rebin_combine(const RebinTaps& t, const GatherEntry& e) {
   const float lo = fmaf(e.fa, t.c0 - t.a0, t.a0);
   const float hi = fmaf(e.fa, t.c1 - t.a1, t.a1);
   return fmaf(e.fb, hi - lo, lo);
 }
-__device__ __forceinline__ float rebin_combine_finite(const RebinTaps& t, const RebinDev& e) {
+__device__ __forceinline__ float rebin_combine_finite(const RebinTaps& t, const GatherEntry& e) {
   return rebin_combine(RebinTaps{finite_or_zero(t.a0), finite_or_zero(t.a1),
                                  finite_or_zero(t.c0), finite_or_zero(t.c1)},
                        e);
@@ -138,7 +165,7 @@ __device__ __forceinline__ float rebin_combine_finite(const RebinTaps& t, const 
 // plain combination non-finite); 0 past ncols and without a fan sample
 // (cuda_kernels.cuh:32-34).
 template <int K>
-__device__ __forceinline__ void rebin_values(const RebinTaps* tp, const RebinDev* const* e,
+__device__ __forceinline__ void rebin_values(const RebinTaps* tp, const GatherEntry* const* e,
                                              const bool* in, float* v) {
   bool finite = true;
 #pragma unroll
@@ -151,7 +178,7 @@ __device__ __forceinline__ void rebin_values(const RebinTaps* tp, const RebinDev
     for (int k = 0; k < K; ++k) v[k] = rebin_combine_finite(tp[k], *e[k]);
   }
 #pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = in[k] && e[k]->state ? v[k] : 0.0f;
+  for (int k = 0; k < K; ++k) v[k] = in[k] && e[k]->valid ? v[k] : 0.0f;
 }
 
 // Rebinned samples of the pair's lines (j0, q0) and (j1, q1) at channels
@@ -164,18 +191,19 @@ __device__ __forceinline__ void rebin_values(const RebinTaps* tp, const RebinDev
 template <bool ZFFS>
 __device__ __forceinline__ void gather_pair(const RebinFilterArgs& a, int t, int T, int j0,
                                             int q0, int j1, int q1, bool has1, float va[4],
-                                            float vb[4]) {
+                                            float vb[4], const uint4* ents) {
   const RawWindow w = raw_window(a, j0);
   const uint32_t rstride = (uint32_t)a.raw_ncols * (uint32_t)sizeof(float);
   const uint32_t rowB0 = (uint32_t)(q0 >> a.zshift) * rstride;
   const uint32_t rowB1 = (uint32_t)(q1 >> a.zshift) * rstride;
   const int jr0 = j0 - w.m_lo, jr1 = (has1 ? j1 : j0) - w.m_lo;
-  RebinDev e[4];
+  GatherEntry e[4];
   bool in[8];
-  const RebinDev* ep[8];
+  const GatherEntry* ep[8];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    e[m] = rebin_entry_of(a, q0, min(t + m * T, a.ncols - 1));
+    e[m] = ZFFS ? gather_entry(rebin_entry_of(a, q0, min(t + m * T, a.ncols - 1)))
+                : unpack_entry(ents[m * T + t]);
     in[m] = t + m * T < a.ncols;
     in[m + 4] = has1 && in[m];
     ep[m] = ep[m + 4] = &e[m];
@@ -202,7 +230,7 @@ __device__ __forceinline__ void gather_pair(const RebinFilterArgs& a, int t, int
   rebin_values<4>(tp, ep, in, va);
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    e[m] = rebin_entry_of(a, q1, min(t + m * T, a.ncols - 1));
+    e[m] = gather_entry(rebin_entry_of(a, q1, min(t + m * T, a.ncols - 1)));
     tp[m] = rebin_taps_w(w, jr1, rowB1, e[m], a.group_bytes);
   }
   rebin_values<4>(tp, ep + 4, in + 4, vb);
@@ -543,8 +571,15 @@ void rebin_filter_reg_kernel(RebinFilterArgs a) {
   constexpr int kSlots = N + N / 8;
   __shared__ FctCplx lds[kPingPong ? 2 * kSlots : kSlots];
   __shared__ float edge[T / 64][4][2];   // lane 0 of each wave: FP16 / BF16 neighbours
+  __shared__ uint4 ents[4 * T];          // the thread's packed table entries (no z FFS)
   const int t = threadIdx.x;
   const FctCplx* tw = reinterpret_cast<const FctCplx*>(a.tw);
+  // Each thread packs and later reads only its own slots (no barrier).
+  if (!a.zshift && !a.lines_in) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      ents[m * T + t] = pack_entry(a.tab[min(t + m * T, a.ncols - 1)]);
+  }
 
   FctCplx tw1[8], tw2[8], tw3[8];
   load_twiddles<N, 1>(tw1, tw, t);
@@ -608,9 +643,9 @@ void rebin_filter_reg_kernel(RebinFilterArgs a) {
       // give 0 (cuda_kernels.cuh:32-34).
       float va[4], vb[4];
       if (a.zshift)
-        gather_pair<true>(a, t, T, j0, r0, j1, r1, has1, va, vb);
+        gather_pair<true>(a, t, T, j0, r0, j1, r1, has1, va, vb, ents);
       else
-        gather_pair<false>(a, t, T, j0, r0, j1, r1, has1, va, vb);
+        gather_pair<false>(a, t, T, j0, r0, j1, r1, has1, va, vb, ents);
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         v[m] = FctCplx{va[m], vb[m]};
