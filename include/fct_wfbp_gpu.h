@@ -182,7 +182,10 @@ FCT_API const char* fct_wfbp_gpu_last_error(const fct_wfbp_gpu_ctx* ctx);
  * (ReconstructionMachine.cu:44-139, 478-544, recon_structs.h:69-83).
  * FP64 view tables and slice grids, FP32 per-chunk offsets, filter spectrum,
  * rebinning tables.  stream is the hipStream_t the plan's work is queued on;
- * NULL is the device's null stream. */
+ * NULL is the device's null stream.  FCT_ERR_LIMIT for scans outside the
+ * rebinning's addressing: a raw view group (focal states x rows x channels
+ * floats) of 16 MiB or more, a fan spanning 4 GiB of raw views, fan view
+ * offsets beyond -2048..2047 or 2^20 or more projections per focal state. */
 FCT_API int fct_wfbp_gpu_plan_create(fct_wfbp_gpu_ctx* ctx, const fct_scan_geometry* scan,
                                      const fct_recon_config* recon,
                                      const fct_exec_options* opts, void* stream,
