@@ -89,13 +89,7 @@ __device__ __forceinline__ RawWindow raw_window(const RebinFilterArgs& a, int j0
   return w;
 }
 
-__device__ __forceinline__ float raw_tap(const RawWindow& w, uint32_t off) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(w.rsrc, (int)off, 0, 0));
-}
-
-// rebin_sample through the window: the same clamped views and taps.  jr =
-// j - m_lo; rowB = byte offset of the raw row.  tap[] = (a0, a1, c0, c1):
-// views ma / mb at fan channels b0 / b1.
+// The four taps of rebin_sample: views ma / mb at fan channels b0 / b1.
 struct RebinTaps {
   float a0, a1, c0, c1;
 };
@@ -127,10 +121,12 @@ __device__ __forceinline__ GatherEntry unpack_entry(uint4 p) {
                      __uint_as_float(p.w), (p.x & 1u) != 0, (p.x & 2u) != 0};
 }
 
-// The two fan channels of a tap row are adjacent (b1 = b0 + 1) except at the
-// last fan channel (b1 = b0): one 8-byte load per view reads both, starting
-// at b0, or at b0 - 1 for the last channel (inside the row), whose second
-// element is then the tap for both -- the same values as four single loads.
+// rebin_sample's taps through the window (jr = j - m_lo; rowB = byte offset
+// of the raw row): the same clamped views.  The two fan channels of a tap
+// row are adjacent (b1 = b0 + 1) except at the last fan channel (b1 = b0):
+// one 8-byte load per view reads both, starting at b0, or at b0 - 1 for the
+// last channel (inside the row: plans have at least two channels), whose
+// second element is then the tap for both -- the values of four single loads.
 __device__ __forceinline__ RebinTaps rebin_taps_w(const RawWindow& w, int jr, uint32_t rowB,
                                                   const GatherEntry& e, uint32_t group_bytes) {
   const int x = jr + e.aoff;
