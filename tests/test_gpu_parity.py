@@ -393,6 +393,28 @@ def test_error_paths(ctx):
         p.backproject(1, None, 1)
 
 
+def test_rebinning_limits_are_errors(ctx):
+    # Scans outside the rebinning's addressing are refused at plan creation
+    # (FCT_ERR_LIMIT, before any device buffer is allocated): fan view offsets
+    # beyond the packed table's -2048..2047 (a 64-channel fan of +-0.6 rad
+    # sampled 30000 times per rotation: offsets near +-2900), and a raw view
+    # group of 16 MiB (2048 rows x 2048 channels of floats).
+    from freect_amd import wfbp
+    w = S.tiny()
+    sc = w.scan
+    wide = sc.replace(n_channels=64, central_channel=31.5, channel_spacing=0.6 / 32 * sc.dsd,
+                      views_per_rotation=30000, n_views=30000)
+    with pytest.raises(wfbp.FctError, match="packed form"):
+        wfbp.Plan(ctx, wide, w.recon)
+    big = sc.replace(n_rows=2048, central_row=1023.5, row_spacing=0.01, n_channels=2048,
+                     central_channel=1023.5, channel_spacing=0.3, views_per_rotation=16,
+                     n_views=20)
+    with pytest.raises(wfbp.FctError, match="raw view group"):
+        wfbp.Plan(ctx, big, w.recon)
+    # A plan within the limits still works after the refusals.
+    wfbp.Plan(ctx, sc, w.recon)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp16"])
 def test_non_finite_raw_reads_as_zero(ctx, precision):
     # A dead (NaN) or saturated (+/-Inf) detector reading enters the
